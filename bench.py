@@ -1,0 +1,282 @@
+"""Throughput of the fused collide-and-stream-pull step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints ONE JSON line on stdout.  A step is one fused kernel launch over the
+whole lattice (inputs resident in HBM).  `value` is MLUPS = cells * K / t
+(wall clock of the K timed steps, device synchronised on both sides).  The
+roofline block divides the algorithmic bytes per launch (2 Q sizeof(storage)
+per cell, plus the alpha state for Newton ELBM) by the average kernel
+duration measured with HIP events on the stream the kernels run on.  The CPU
+baseline is the FP64 oracle (a restatement of the reference's CPU loop,
+OpenMP over x) timed on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS + % HBM roofline; entropic MLUPS learned-α vs Newton at equal residual"
+PEAK_HBM_GBS = 8000.0
+
+# name -> (description, config kwargs, init, bytes per storage value)
+WORKLOADS = {
+    "d3q19-bgk-256-f32": dict(
+        desc="D3Q19 SRT BGK 256^3 periodic, SoA two-lattice pull, FP32 shifted storage, "
+             "FP32 compute, tau 0.55, Guo + constant F=(1e-6,0,0), rho0=1, u0 noise +-1e-3 (seed 42)",
+        cfg=dict(lattice="D3Q19", nx=256, ny=256, nz=256, tau=0.55, forcing_scheme="Guo",
+                 force="Constant", amplitude=(1e-6, 0, 0), storage="F32", compute="F32",
+                 shifted=True),
+        init=("noise", 1e-3, 42), steps=200),
+    "d3q19-bgk-256-f64": dict(
+        desc="D3Q19 SRT BGK 256^3 periodic, FP64 storage and compute (reference order), "
+             "tau 0.55, Guo + constant F=(1e-6,0,0), u0 noise +-1e-3 (seed 42)",
+        cfg=dict(lattice="D3Q19", nx=256, ny=256, nz=256, tau=0.55, forcing_scheme="Guo",
+                 force="Constant", amplitude=(1e-6, 0, 0), storage="F64", compute="F64",
+                 shifted=False),
+        init=("noise", 1e-3, 42), steps=200),
+    "d3q27-bgk-384-f16": dict(
+        desc="D3Q27 SRT BGK 384^3 Taylor-Green U0=0.05, tau 0.51, FP32 compute, shifted FP16 storage",
+        cfg=dict(lattice="D3Q27", nx=384, ny=384, nz=384, tau=0.51, forcing_scheme="None_",
+                 force="None_", storage="F16", compute="F32", shifted=True),
+        init=("tgv", 0.05), steps=200),
+    "d3q27-bgk-384-f32": dict(
+        desc="D3Q27 SRT BGK 384^3 Taylor-Green U0=0.05, tau 0.51, FP32 compute, shifted FP32 storage",
+        cfg=dict(lattice="D3Q27", nx=384, ny=384, nz=384, tau=0.51, forcing_scheme="None_",
+                 force="None_", storage="F32", compute="F32", shifted=True),
+        init=("tgv", 0.05), steps=200),
+    "d3q27-bgk-384-f64": dict(
+        desc="D3Q27 SRT BGK 384^3 Taylor-Green U0=0.05, tau 0.51, FP32 compute, shifted FP64 storage",
+        cfg=dict(lattice="D3Q27", nx=384, ny=384, nz=384, tau=0.51, forcing_scheme="None_",
+                 force="None_", storage="F64", compute="F32", shifted=True),
+        init=("tgv", 0.05), steps=100),
+    "d2q9-elbm-2048-newton": dict(
+        desc="D2Q9 SRT Entropic 2048^2, nu=1e-5 (tau=0.50003), Kolmogorov k=4 A=1e-5, "
+             "u0 noise +-1e-3 (seed 42), FP32 compute + FP64 Newton alpha solve",
+        cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
+                 forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
+                 wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True),
+        init=("noise", 1e-3, 42), steps=200),
+    "d2q9-bgk-128-f64": dict(
+        desc="D2Q9 SRT BGK 128^2 FP64, tau 0.6, Guo + Kolmogorov F_x=1e-5 sin(2 pi 4 y/128), "
+             "u0 noise +-1e-3 (seed 42) -- cache-resident parity config",
+        cfg=dict(lattice="D2Q9", nx=128, ny=128, nz=1, tau=0.6, forcing_scheme="Guo",
+                 force="Kolmogorov", amplitude=(1e-5, 0, 0), wavelength=(32.0, 1, 1),
+                 storage="F64", compute="F64", shifted=False),
+        init=("noise", 1e-3, 42), steps=1000),
+}
+DEFAULT = "d3q19-bgk-256-f32"
+SIZES = {"F64": 8, "F32": 4, "F16": 2}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Hip:
+    """The few HIP runtime calls bench.py needs (events and one stream)."""
+
+    def __init__(self):
+        for p in ("/opt/rocm/lib/libamdhip64.so", "libamdhip64.so"):
+            try:
+                self.L = ctypes.CDLL(p)
+                break
+            except OSError:
+                continue
+        else:
+            raise RuntimeError("libamdhip64.so not found")
+        vp = ctypes.c_void_p
+        self.L.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+        self.L.hipEventRecord.argtypes = [vp, vp]
+        self.L.hipEventSynchronize.argtypes = [vp]
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+        self.L.hipEventDestroy.argtypes = [vp]
+        self.L.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+        self.L.hipStreamSynchronize.argtypes = [vp]
+        self.L.hipStreamDestroy.argtypes = [vp]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self.ok(self.L.hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
+        return s
+
+    def event(self):
+        e = ctypes.c_void_p()
+        self.ok(self.L.hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e, s):
+        self.ok(self.L.hipEventRecord(e, s), "hipEventRecord")
+
+    def elapsed_ms(self, a, b):
+        self.ok(self.L.hipEventSynchronize(b), "hipEventSynchronize")
+        t = ctypes.c_float()
+        self.ok(self.L.hipEventElapsedTime(ctypes.byref(t), a, b), "hipEventElapsedTime")
+        return float(t.value)
+
+    def sync(self):
+        self.ok(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+def make_config(spec):
+    import metalbm_amd as m
+    kw = dict(spec)
+    conv = {"lattice": m.LatticeType, "collision": m.CollisionType,
+            "forcing_scheme": m.ForcingSchemeType, "force": m.ForceType,
+            "equilibrium": m.EquilibriumType, "storage": m.Precision, "compute": m.Precision}
+    for k, e in conv.items():
+        if k in kw and isinstance(kw[k], str):
+            kw[k] = e[kw[k]]
+    return m.Config(**kw)
+
+
+def init(a, spec):
+    kind = spec[0]
+    if kind == "noise":
+        a.init_noise(1.0, spec[1], spec[2])
+    elif kind == "tgv":
+        a.init_taylor_green(1.0, spec[1])
+    else:
+        raise ValueError(kind)
+
+
+def bytes_per_cell(cfg, wl_cfg) -> int:
+    q = cfg.q
+    b = 2 * q * SIZES[wl_cfg["storage"]]
+    if wl_cfg.get("collision") == "ELBM":
+        b += 2 * (8 if wl_cfg["compute"] == "F64" else 4)   # alpha warm-start state r+w
+    return b
+
+
+def cpu_baseline(wl, max_seconds=15.0):
+    """The FP64 oracle on a bounded sample of the same workload (host cores)."""
+    import numpy as np
+
+    from oracle import oracle as o
+    o.build()
+    c = wl["cfg"]
+    names = {"D2Q9": 3, "D3Q19": 9, "D3Q27": 10}
+    cfg = o.Config(lattice=names[c["lattice"]],
+                   collision=2 if c.get("collision") == "ELBM" else 1,
+                   forcing_scheme={"None_": 1, "Guo": 2, "ShanChen": 3,
+                                   "ExactDifferenceMethod": 4}[c.get("forcing_scheme", "Guo")],
+                   force={"None_": 0, "Constant": 4, "Sinusoidal": 5,
+                          "Kolmogorov": 6}[c.get("force", "None_")],
+                   equilibrium=3, nx=c["nx"], ny=c["ny"], nz=c.get("nz", 1), tau=c["tau"],
+                   amplitude=c.get("amplitude", (0, 0, 0)),
+                   wavelength=c.get("wavelength", (1, 1, 1)))
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    o.lib().ref_set_threads(threads)
+    if wl["init"][0] == "noise":
+        u = o.noise_velocity(cfg, wl["init"][2], wl["init"][1])
+    else:
+        u = o.taylor_green(cfg, wl["init"][1])
+    f = o.init_equilibrium(cfg, np.ones(cfg.cells), u)
+    del u
+    t0 = time.perf_counter()
+    o.run(cfg, f, 1, threads=threads)
+    t1 = time.perf_counter() - t0
+    n = int(max(1, min(50, (max_seconds - t1) / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    o.run(cfg, f, n, threads=threads)
+    t = time.perf_counter() - t0
+    return {"value": round(cfg.cells * n / t / 1e6, 3), "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 oracle (OpenMP over x, {threads} threads), {n} steps of the "
+                      f"same {cfg.nx}x{cfg.ny}x{cfg.nz} workload after 1 warm-up step"}
+
+
+def traffic_of(name):
+    p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch")
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build runs one process-local domain "
+              "on one MI355X (no halo exchange); run it with --gpus 1", file=sys.stderr)
+        return 2
+    wl = WORKLOADS[args.workload]
+    K = args.steps if args.steps is not None else wl["steps"]
+    W = args.warmup
+
+    import metalbm_amd as m
+    hip = Hip()
+    cfg = make_config(wl["cfg"])
+    log(f"workload {args.workload}: {wl['desc']}")
+    a = m.Algorithm(cfg)
+    s = hip.stream()
+    a.set_stream(s.value)
+    init(a, wl["init"])
+    a.iterate(W)
+    a.synchronize()
+    e0, e1 = hip.event(), hip.event()
+    hip.sync()
+    t0 = time.perf_counter()
+    hip.record(e0, s)
+    a.iterate(K)
+    hip.record(e1, s)
+    a.synchronize()
+    hip.sync()
+    wall = time.perf_counter() - t0
+    ev_ms = hip.elapsed_ms(e0, e1)
+    N = cfg.cells
+    mlups = N * K / wall / 1e6
+    kern_s = ev_ms / 1e3 / K
+    bpc = bytes_per_cell(cfg, wl["cfg"])
+    achieved = bpc * N / kern_s / 1e9
+    log(f"wall {wall * 1e3:.2f} ms for {K} steps, events {ev_ms:.2f} ms, "
+        f"{mlups:.1f} MLUPS, {achieved:.1f} GB/s algorithmic")
+    stats = a.elbm_stats() if wl["cfg"].get("collision") == "ELBM" else None
+    a.close()
+    out = {
+        "metric": METRIC, "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(wall * 1e3 / K, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"F64": "f64", "F32": "f32", "F16": "f16"}[wl["cfg"]["storage"]] + "-storage/"
+                 + {"F64": "f64", "F32": "f32"}[wl["cfg"]["compute"]] + "-compute",
+        "data": "synthetic (seeded initial field, see config.init)",
+        "config": {"workload": args.workload, "description": wl["desc"],
+                   "cells": N, "bytes_per_cell_update": bpc,
+                   "init": list(wl["init"]), "parallelism": "single GPU, one domain"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic_of(args.workload),
+                     "kernel_avg_us": round(kern_s * 1e6, 2)},
+    }
+    if stats:
+        out["elbm_stats"] = stats
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(wl)
+        except Exception as e:  # the baseline is a report, not the measurement
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,178 @@
+/*
+ * metalbm_amd.h -- C ABI of the MI355X-native fused collide-and-stream-pull
+ * step (libmetalbm_amd.so).
+ *
+ * The entry points replace, for one GPU and one process-local periodic
+ * domain, the reference's per-step operator chain:
+ *   Algorithm<...>::iterate            include/metaLBM/Algorithm.h:326-358
+ *   Algorithm<Generic>::operator()     include/metaLBM/Algorithm.h:97-126
+ *   Collision_ / ForcingScheme_ / Force_ / Equilibrium_ template plugins
+ *                                      include/metaLBM/Collision.h:913-914,
+ *                                      ForcingScheme.h:200, Force.h:628,
+ *                                      Equilibrium.h:330
+ *   initDistribution                   include/metaLBM/Initialize.h:91-117
+ *   storeFields                        include/metaLBM/Algorithm.h:150-194
+ *   TotalEnergy / reduce               include/metaLBM/Analysis.h:33-98,
+ *                                      Communication.h:76-101
+ * Plain C types only; every call returns LBM_OK (0) or a negative LBM_E*
+ * code and records a message readable with lbm_last_error().  Nothing ever
+ * calls exit() (the reference exits, Commons.h:6-38).
+ *
+ * Layout of host arrays: populations f[Q][N] (population-major SoA), vector
+ * fields u[D][N], scalars rho[N]; cell = (x*ny + y)*nz + z in 3D (z fastest)
+ * and x*ny + y in 2D (y fastest) -- the reference's halo-space order
+ * (Domain.h:204-207, 272-276) without halos or FFTW padding.
+ *
+ * Threading: a handle belongs to one host thread.  All device work is
+ * enqueued on the handle's HIP stream; calls that return data to the host
+ * synchronise that stream, lbm_step does not.
+ */
+#ifndef METALBM_AMD_H
+#define METALBM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+/* Error codes. */
+enum {
+  LBM_OK = 0,
+  LBM_E_INVALID = -1,   /* bad argument or unsupported configuration */
+  LBM_E_HIP = -2,       /* HIP runtime error (message has the HIP text) */
+  LBM_E_NOMEM = -3,     /* device allocation failed */
+  LBM_E_STATE = -4      /* call not valid in the handle's current state */
+};
+
+/* Enumerator values equal the reference's enum classes
+ * (include/metaLBM/Options.h:12-41). */
+enum { LBM_D2Q9 = 3, LBM_D3Q19 = 9, LBM_D3Q27 = 10 };             /* LatticeType */
+enum { LBM_BGK = 1, LBM_ELBM = 2 };                                /* CollisionType */
+enum { LBM_FS_NONE = 1, LBM_FS_GUO = 2, LBM_FS_SHANCHEN = 3,
+       LBM_FS_EDM = 4 };                                           /* ForcingSchemeType */
+enum { LBM_FORCE_NONE = 0, LBM_FORCE_CONSTANT = 4,
+       LBM_FORCE_SINUSOIDAL = 5, LBM_FORCE_KOLMOGOROV = 6 };       /* ForceType */
+enum { LBM_EQ_EXACT = 1, LBM_EQ_MA3 = 3 };                         /* EquilibriumType */
+
+/* Arithmetic types (storage of populations, compute of the step). */
+enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16 = 2 };
+
+/* Entropic alpha solver (ELBM only). */
+enum {
+  LBM_ALPHA_NEWTON = 0,   /* reference Newton: warm start from the stored
+                             alpha, |step| <= 1e-8, <= 50 iterations
+                             (Collision.h:328-375, EntropicStep.h:111-140) */
+  LBM_ALPHA_LEARNED = 1   /* MLP guess + one Newton correction, certified by
+                             the H-residual bound; Newton fallback */
+};
+
+typedef struct {
+  int lattice;          /* LBM_D2Q9 | LBM_D3Q19 | LBM_D3Q27 */
+  int collision;        /* LBM_BGK | LBM_ELBM */
+  int forcing_scheme;   /* LBM_FS_* */
+  int force;            /* LBM_FORCE_* */
+  int equilibrium;      /* LBM_EQ_MA3 | LBM_EQ_EXACT (not for D3Q19) */
+  int nx, ny, nz;       /* nz == 1 for D2Q9; nx*ny*nz < 2^31 */
+  double tau;           /* relaxationTime (Input_prod.in:38) */
+  double amplitude[3];  /* forceAmplitude (Input_prod.in:50) */
+  double wavelength[3]; /* forceWaveLength (Input_prod.in:51) */
+  int storage;          /* LBM_F64 | LBM_F32 | LBM_F16 (F16 needs shifted) */
+  int shifted;          /* nonzero: store f_i - w_i instead of f_i */
+  int compute;          /* LBM_F64 | LBM_F32 */
+  int alpha_mode;       /* LBM_ALPHA_* (ELBM) */
+  double residual_tol;  /* H-residual tolerance of LBM_ALPHA_LEARNED */
+} lbm_config;
+
+typedef struct lbm_handle lbm_handle;
+
+/* Per-run entropic statistics (cells per branch, summed over steps). */
+typedef struct {
+  int64_t small;        /* max|fneq/f| <= 1e-3, alpha = 2      (Collision.h:356-359) */
+  int64_t capped;       /* alphaMax < 2, alpha = 0.95 alphaMax (Collision.h:365-367) */
+  int64_t newton;       /* Newton converged inside (1, alphaMax) */
+  int64_t fallback;     /* Newton failed or out of bounds, alpha = 2 (Collision.h:344-346) */
+  int64_t newton_iterations;
+  int64_t learned_accepted;  /* LEARNED: cells certified after one correction */
+} lbm_elbm_stats;
+
+int lbm_abi_version(void);
+
+/* Fills a config with the reference defaults: D2Q9 BGK, Guo, no force,
+ * Ma3, 16x16, tau 0.6, F64 storage and compute, unshifted. */
+int lbm_config_default(lbm_config *cfg);
+
+/* Creates a handle on the current HIP device: validates cfg, allocates the
+ * two population lattices (ping-pong) and, for ELBM, the alpha field set to 2
+ * (Initialize.h:83-89).  Replaces Routine's constructor chain
+ * (Routine.h:58-84) minus I/O. */
+int lbm_create(const lbm_config *cfg, lbm_handle **out);
+int lbm_destroy(lbm_handle *h);
+
+/* Message of the last failed call on h (or of the last failed lbm_create
+ * when h is NULL).  Valid until the next call on h. */
+const char *lbm_last_error(const lbm_handle *h);
+
+/* Sizes derived from the configuration. */
+int lbm_sizes(const lbm_handle *h, int64_t *cells, int *q, int *d);
+
+/* f = feq(rho, u) (Initialize.h:106-117).  rho[N], u[D][N] on the host. */
+int lbm_init_equilibrium(lbm_handle *h, const double *rho, const double *u);
+
+/* f = feq(rho0, u0) with u0 seeded uniform noise in [-amp, amp) per
+ * component: u_d(cell) = amp (2 U - 1), U = (SplitMix64(seed, cell*D + d)
+ * >> 11) 2^-53.  Generated on the device. */
+int lbm_init_noise(lbm_handle *h, double rho0, double amp, uint64_t seed);
+
+/* f = feq(rho0, u0) with the Taylor-Green vortex u0 = U0 (sin x cos y cos z,
+ * -cos x sin y cos z, 0) (2D: without the z factor), x = 2 pi i / n. */
+int lbm_init_taylor_green(lbm_handle *h, double rho0, double u0);
+
+/* Populations as FP64 f_i (unshifted) on the host, [Q][N]. */
+int lbm_set_populations(lbm_handle *h, const double *f);
+int lbm_get_populations(lbm_handle *h, double *f);
+
+/* The per-cell alpha field (ELBM warm start; Algorithm.h:103-106). */
+int lbm_set_alpha(lbm_handle *h, const double *alpha);
+int lbm_get_alpha(lbm_handle *h, double *alpha);
+
+/* Advances nsteps fused collide-and-stream-pull steps (Routine.h:121-147,
+ * Algorithm.h:326-358).  If store_last is nonzero the last step also writes
+ * the output fields rho, u_hydro, alpha, force as storeFields does
+ * (Algorithm.h:150-194).  Asynchronous on the handle's stream. */
+int lbm_step(lbm_handle *h, int nsteps, int store_last);
+
+/* Copies the fields written by the last storing step: rho[N],
+ * u_hydro[D][N], alpha[N], force[D][N]; any pointer may be NULL. */
+int lbm_get_fields(lbm_handle *h, double *rho, double *u_hydro,
+                   double *alpha, double *force);
+
+/* FP64 diagnostics of the stored fields, deterministic two-pass reduction:
+ * mass = sum rho; energy = (1/N) sum 1/2 rho |u_h|^2 (Analysis.h:53-61);
+ * enstrophy = (1/N) sum 1/2 |curl u_h|^2 with second-order central
+ * differences and periodic wrap.  Any output pointer may be NULL. */
+int lbm_reduce(lbm_handle *h, double *mass, double *energy, double *enstrophy);
+
+/* Entropic statistics accumulated since creation or the last reset. */
+int lbm_get_elbm_stats(lbm_handle *h, lbm_elbm_stats *out);
+int lbm_reset_elbm_stats(lbm_handle *h);
+
+/* Stream control: the handle enqueues on `stream` (a hipStream_t, may be
+ * NULL for the null stream) from now on. */
+int lbm_set_stream(lbm_handle *h, void *stream);
+int lbm_synchronize(lbm_handle *h);
+
+/* Device pointer of the lattice holding the current populations and its
+ * element size, for zero-copy callers. */
+int lbm_device_populations(lbm_handle *h, void **ptr, int *elem_bytes);
+
+/* Steps taken since creation. */
+int64_t lbm_steps_done(const lbm_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* METALBM_AMD_H */

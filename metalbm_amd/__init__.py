@@ -1,0 +1,252 @@
+"""metalbm_amd -- MI355X-native fused collide-and-stream-pull step of metaLBM.
+
+Host-side mirror of the reference's operator vocabulary over the C ABI of
+libmetalbm_amd.so (include/metalbm_amd.h):
+
+* enums with the reference's names and integer values
+  (include/metaLBM/Options.h:12-41): ``LatticeType``, ``CollisionType``,
+  ``ForcingSchemeType``, ``ForceType``, ``EquilibriumType``;
+* ``Config`` -- the run parameters the reference takes from its compile-time
+  input file (src/Input_prod.in:9-79) plus storage/compute precision;
+* ``Algorithm`` -- one periodic domain on the GPU; ``iterate(n)`` advances n
+  fused steps (Algorithm.h:326-358, Routine.h:121-147), ``store_fields``
+  returns rho, u_hydro, alpha, force as storeFields writes them
+  (Algorithm.h:150-194).
+
+Errors raise :class:`LBMError` (the reference prints and exits,
+Commons.h:6-38).  All numerics run in the HIP library; this module only moves
+numpy arrays across the ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+from dataclasses import dataclass
+
+import numpy as np
+
+from .capi import LBMError, LbmConfigC, LbmElbmStatsC, check, lib
+
+__all__ = [
+    "LatticeType", "CollisionType", "ForcingSchemeType", "ForceType",
+    "EquilibriumType", "Precision", "AlphaMode", "Config", "Algorithm",
+    "LBMError", "lattice_q", "lattice_d",
+]
+
+
+class LatticeType(enum.IntEnum):
+    D2Q9 = 3
+    D3Q19 = 9
+    D3Q27 = 10
+
+
+class CollisionType(enum.IntEnum):
+    BGK = 1
+    ELBM = 2
+
+
+class ForcingSchemeType(enum.IntEnum):
+    None_ = 1
+    Guo = 2
+    ShanChen = 3
+    ExactDifferenceMethod = 4
+
+
+class ForceType(enum.IntEnum):
+    None_ = 0
+    Constant = 4
+    Sinusoidal = 5
+    Kolmogorov = 6
+
+
+class EquilibriumType(enum.IntEnum):
+    Exact = 1
+    TruncationMa3 = 3
+
+
+class Precision(enum.IntEnum):
+    F64 = 0
+    F32 = 1
+    F16 = 2
+
+
+class AlphaMode(enum.IntEnum):
+    NEWTON = 0
+    LEARNED = 1
+
+
+_QD = {LatticeType.D2Q9: (9, 2), LatticeType.D3Q19: (19, 3), LatticeType.D3Q27: (27, 3)}
+
+
+def lattice_q(lattice) -> int:
+    return _QD[LatticeType(lattice)][0]
+
+
+def lattice_d(lattice) -> int:
+    return _QD[LatticeType(lattice)][1]
+
+
+@dataclass
+class Config:
+    lattice: int = LatticeType.D2Q9
+    collision: int = CollisionType.BGK
+    forcing_scheme: int = ForcingSchemeType.Guo
+    force: int = ForceType.None_
+    equilibrium: int = EquilibriumType.TruncationMa3
+    nx: int = 16
+    ny: int = 16
+    nz: int = 1
+    tau: float = 0.6
+    amplitude: tuple = (0.0, 0.0, 0.0)
+    wavelength: tuple = (1.0, 1.0, 1.0)
+    storage: int = Precision.F64
+    shifted: bool = False
+    compute: int = Precision.F64
+    alpha_mode: int = AlphaMode.NEWTON
+    residual_tol: float = 1e-10
+
+    def to_c(self) -> LbmConfigC:
+        c = LbmConfigC()
+        for name in ("lattice", "collision", "forcing_scheme", "force",
+                     "equilibrium", "nx", "ny", "nz", "storage", "compute",
+                     "alpha_mode"):
+            setattr(c, name, int(getattr(self, name)))
+        c.shifted = 1 if self.shifted else 0
+        c.tau = float(self.tau)
+        c.residual_tol = float(self.residual_tol)
+        for d in range(3):
+            c.amplitude[d] = float(self.amplitude[d])
+            c.wavelength[d] = float(self.wavelength[d])
+        return c
+
+    @property
+    def q(self) -> int:
+        return lattice_q(self.lattice)
+
+    @property
+    def d(self) -> int:
+        return lattice_d(self.lattice)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _f64(a, shape) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != int(np.prod(shape)):
+        raise LBMError(-1, f"array of size {a.size} where {shape} was expected")
+    return a.reshape(shape)
+
+
+class Algorithm:
+    """One periodic LBM domain resident on the current GPU."""
+
+    def __init__(self, config: Config):
+        self.config = config
+        self._h = ctypes.c_void_p()
+        c = config.to_c()
+        check(lib().lbm_create(ctypes.byref(c), ctypes.byref(self._h)), None)
+        self.N, self.Q, self.D = config.cells, config.q, config.d
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            lib().lbm_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        check(rc, self._h)
+
+    # -- initial state ---------------------------------------------------------
+    def init_equilibrium(self, rho, u) -> None:
+        """f = feq(rho, u) (Initialize.h:106-117); rho[N], u[D, N]."""
+        r = _f64(rho, (self.N,))
+        v = _f64(u, (self.D, self.N))
+        self._check(lib().lbm_init_equilibrium(self._h, _dp(r), _dp(v)))
+
+    def init_noise(self, rho0: float, amplitude: float, seed: int) -> None:
+        self._check(lib().lbm_init_noise(self._h, rho0, amplitude, seed))
+
+    def init_taylor_green(self, rho0: float, u0: float) -> None:
+        self._check(lib().lbm_init_taylor_green(self._h, rho0, u0))
+
+    # -- populations / fields --------------------------------------------------
+    def set_populations(self, f) -> None:
+        a = _f64(f, (self.Q, self.N))
+        self._check(lib().lbm_set_populations(self._h, _dp(a)))
+
+    def populations(self) -> np.ndarray:
+        f = np.empty((self.Q, self.N))
+        self._check(lib().lbm_get_populations(self._h, _dp(f)))
+        return f
+
+    def set_alpha(self, alpha) -> None:
+        a = _f64(alpha, (self.N,))
+        self._check(lib().lbm_set_alpha(self._h, _dp(a)))
+
+    def alpha(self) -> np.ndarray:
+        a = np.empty(self.N)
+        self._check(lib().lbm_get_alpha(self._h, _dp(a)))
+        return a
+
+    def iterate(self, nsteps: int = 1, store: bool = False) -> None:
+        """Advance nsteps fused steps; store=True writes the output fields on
+        the last one (asynchronous, like a kernel launch)."""
+        self._check(lib().lbm_step(self._h, int(nsteps), 1 if store else 0))
+
+    def synchronize(self) -> None:
+        self._check(lib().lbm_synchronize(self._h))
+
+    def store_fields(self):
+        """(rho[N], u_hydro[D, N], alpha[N], force[D, N]) of the last storing step."""
+        rho = np.empty(self.N)
+        u = np.empty((self.D, self.N))
+        al = np.empty(self.N)
+        F = np.empty((self.D, self.N))
+        self._check(lib().lbm_get_fields(self._h, _dp(rho), _dp(u), _dp(al), _dp(F)))
+        return rho, u, al, F
+
+    def reduce(self):
+        """(mass, energy, enstrophy) in FP64 from the stored fields."""
+        m, e, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lbm_reduce(self._h, ctypes.byref(m), ctypes.byref(e),
+                                     ctypes.byref(w)))
+        return m.value, e.value, w.value
+
+    def elbm_stats(self) -> dict:
+        s = LbmElbmStatsC()
+        self._check(lib().lbm_get_elbm_stats(self._h, ctypes.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in LbmElbmStatsC._fields_}
+
+    def reset_elbm_stats(self) -> None:
+        self._check(lib().lbm_reset_elbm_stats(self._h))
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        self._check(lib().lbm_set_stream(self._h, ctypes.c_void_p(stream_ptr or 0)))
+
+    def device_populations(self):
+        p = ctypes.c_void_p()
+        e = ctypes.c_int()
+        self._check(lib().lbm_device_populations(self._h, ctypes.byref(p), ctypes.byref(e)))
+        return p.value, e.value
+
+    @property
+    def steps_done(self) -> int:
+        return int(lib().lbm_steps_done(self._h))

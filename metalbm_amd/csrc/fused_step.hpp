@@ -1,0 +1,294 @@
+// fused_step.hpp -- BGK and entropic (ELBM) fused step kernels.
+//
+// Template parameters:
+//   L   Lattice<...> descriptor              (Lattice.h)
+//   S   storage type of populations          double | float | _Float16
+//   C   compute type                         double | float
+//   FS  collision source                     SRC_ZERO | SRC_GUO | SRC_EDM
+//   EQ  equilibrium                          TruncationMa3 | Exact
+//   SH  populations stored as f_i - w_i
+// When C is double and SH is false every operator runs in the reference's
+// operation order (parity build); otherwise the shifted/FMA form is used.
+#pragma once
+
+#include "step_kernels.hpp"
+
+namespace lbm {
+
+template <class L, class C>
+__device__ __forceinline__ void store_fields(const StepArgs& a, int cell, int FS,
+                                             C rho, const C* u, const C* F,
+                                             double alpha) {
+  a.rho_out[cell] = double(rho);
+  if (FS == (int)ForcingSchemeType::None) {
+    // ForcingScheme<None>::calculateHydrodynamicVelocity (ForcingScheme.h:50-57)
+    for (int d = 0; d < L::dimD; ++d) a.uh_out[(size_t)d * a.N + cell] = double(u[d]);
+  } else {
+    // Generic: u + 0.5 / rho * F (ForcingScheme.h:26-33)
+    const C h = C(0.5) / rho;
+    for (int d = 0; d < L::dimD; ++d)
+      a.uh_out[(size_t)d * a.N + cell] = double(u[d] + F[d] * h);
+  }
+  if (a.alpha_out) a.alpha_out[cell] = alpha;
+  if (a.F_out)
+    for (int d = 0; d < L::dimD; ++d) a.F_out[(size_t)d * a.N + cell] = double(F[d]);
+}
+
+// FSX: the forcing scheme as configured (None vs Shan-Chen matters only for
+// the stored hydrodynamic velocity), passed at run time.
+template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_bgk(const StepArgs a, int FSX) {
+  constexpr int Q = L::dimQ;
+  constexpr bool REF = std::is_same_v<C, double> && !SH;
+  const CellPos c = cell_of_thread<L, BLOCK>(a);
+  if (!c.active) return;
+  const int cell = self_index<L>(a, c);
+  S* __restrict__ dst = static_cast<S*>(a.dst);
+
+  C f[Q];
+  pull<L, S, C>(a, c, f);
+  C F[3];
+  force_of_cell<L, C>(a, c, F);
+  const C of = C(a.omega_f), oe = C(a.omega_eq), gc = C(a.guo_coef);
+  C rho, u[3] = {C(0), C(0), C(0)}, u2;
+
+  if constexpr (REF) {
+    // calculateMoments (Collision.h:72-79)
+    moments_ref<L>(f, rho, u, u2);
+    const RefEquilibrium<L, EQ, C> E(rho, u, u2);
+    C v[3] = {C(0), C(0), C(0)}, v2 = C(0);
+    if constexpr (FS == SRC_EDM) {
+      // ExactDifferenceMethod source: feq(rho, u + F / rho) - feq (ForcingScheme.h:184-197)
+      const C k = C(1.0) / rho;
+      for (int d = 0; d < L::dimD; ++d) v[d] = u[d] + F[d] * k;
+      v2 = v[0] * v[0];
+      for (int d = 1; d < L::dimD; ++d) v2 += v[d] * v[d];
+    }
+    const RefEquilibrium<L, EQ, C> E2(rho, v, v2);
+    // BGK collideAndStream (Collision.h:134-151)
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const C fe = E.template feq<i>();
+      C s = C(0);
+      if constexpr (FS == SRC_GUO) s = guo_ref<L, i>(gc, u, F);
+      if constexpr (FS == SRC_EDM) s = E2.template feq<i>() - fe;
+      const C out = of * f[i] + oe * fe + s;
+      store_pop<L, i, S, C>(dst + (size_t)i * a.N + cell, out);
+    });
+  } else {
+    C drho;
+    if constexpr (SH) {
+      moments_shifted<L>(f, drho, rho, u, u2);
+    } else {
+      moments_ref<L>(f, rho, u, u2);
+      drho = rho - C(1);
+    }
+    const FastEquilibrium<L, EQ, C> E(u, u2);
+    C v[3] = {C(0), C(0), C(0)}, v2 = C(0);
+    if constexpr (FS == SRC_EDM) {
+      const C k = C(1) / rho;
+      for (int d = 0; d < L::dimD; ++d) v[d] = u[d] + F[d] * k;
+      v2 = v[0] * v[0];
+      for (int d = 1; d < L::dimD; ++d) v2 += v[d] * v[d];
+    }
+    const FastEquilibrium<L, EQ, C> E2(v, v2);
+    C uF = u[0] * F[0];
+    for (int d = 1; d < L::dimD; ++d) uF += u[d] * F[d];
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const C w = C(L::W[i]);
+      const C p = E.template p<i>();
+      // feq_i in the storage representation: w (drho + rho p) when shifted.
+      const C eq = SH ? w * (drho + rho * p) : w * rho * (C(1) + p);
+      C s = C(0);
+      if constexpr (FS == SRC_GUO) {
+        const C cF = cdot<L, i>(F);
+        const C cu = cdot<L, i>(u);
+        s = (C(3) * gc * w) * (cF - uF + C(3) * cu * cF);
+      }
+      if constexpr (FS == SRC_EDM) s = rho * w * (E2.template p<i>() - p);
+      const C out = of * f[i] + oe * eq + s;
+      store_pop<L, i, S, C>(dst + (size_t)i * a.N + cell, out);
+    });
+  }
+  if (a.store) store_fields<L, C>(a, cell, FSX, rho, u, F, 2.0);
+}
+
+// ---- entropic collision ------------------------------------------------------
+
+// ELBM alpha (Collision.h:351-375): the deviation test, alphaMax, then the
+// reference Newton-Raphson (EntropicStep.h:111-140) warm-started from the
+// cell's previous alpha.  f and n (= fneq) are the pulled populations and
+// their non-equilibrium parts in FP64.  G and G' share one log per term;
+// their values equal the reference's separate loops (EntropicStep.h:31-62).
+// REFDIV keeps the reference's divisions in the deviation tests.
+template <class L, bool REFDIV>
+__device__ __forceinline__ double elbm_alpha(const double* f, const double* n,
+                                             double guess, int& branch,
+                                             int& iterations) {
+  constexpr int Q = L::dimQ;
+  iterations = 0;
+  bool small = true;
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    if constexpr (REFDIV) {
+      const double dev = fabs(n[i]) / f[i];
+      if (dev > 1.0e-3) small = false;
+    } else {
+      if (!(fabs(n[i]) <= 1.0e-3 * f[i]) && f[i] > 0.0) small = false;
+    }
+  });
+  if (small) {
+    branch = 0;
+    return 2.0;
+  }
+  double amax = 2.5;
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    if (n[i] > 0) {
+      const double t = fabs(f[i]) / n[i];
+      if (t < amax) amax = t;
+    }
+  });
+  if (amax < 2.) {
+    branch = 1;
+    return 0.95 * amax;
+  }
+  double A[Q];
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    A[i] = log(f[i] / L::W[i]);
+  });
+  double x = guess, step = 0.0;
+  bool ok = false;
+  for (int it = 1; it <= 50; ++it) {
+    x = x - step;
+    iterations = it;
+    double g = 0.0, dg = 0.0;
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const double fm = f[i] - x * n[i];
+      const double lg = log(fm / L::W[i]);
+      g += f[i] * A[i] - fm * lg;
+      dg += n[i] * (1 + lg);
+    });
+    step = g / dg;
+    if (fabs(step) <= 1e-8) {
+      ok = (x > 1. && x < amax);
+      break;
+    }
+  }
+  if (!ok) {
+    branch = 3;
+    return 2.0;
+  }
+  branch = 2;
+  return x;
+}
+
+template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
+  constexpr int Q = L::dimQ;
+  constexpr bool REF = std::is_same_v<C, double> && !SH;
+  __shared__ unsigned int s_cnt[6];
+  if (a.stats && threadIdx.x < 6) s_cnt[threadIdx.x] = 0u;
+  const CellPos c = cell_of_thread<L, BLOCK>(a);
+  int branch = -1, its = 0;
+  if (c.active) {
+    const int cell = self_index<L>(a, c);
+    S* __restrict__ dst = static_cast<S*>(a.dst);
+    C* __restrict__ astate = static_cast<C*>(a.alpha_state);
+    C f[Q];
+    pull<L, S, C>(a, c, f);
+    C F[3];
+    force_of_cell<L, C>(a, c, F);
+    const C gc = C(a.guo_coef);
+    C rho, u[3] = {C(0), C(0), C(0)}, u2, drho = C(0);
+    C fneq[Q], eqv[Q];
+    if constexpr (REF) {
+      moments_ref<L>(f, rho, u, u2);
+      const RefEquilibrium<L, EQ, C> E(rho, u, u2);
+      // fneq parked per cell (Collision.h:232-236), here in registers.
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        fneq[i] = f[i] - E.template feq<i>();
+      });
+    } else {
+      if constexpr (SH) {
+        moments_shifted<L>(f, drho, rho, u, u2);
+      } else {
+        moments_ref<L>(f, rho, u, u2);
+        drho = rho - C(1);
+      }
+      const FastEquilibrium<L, EQ, C> E(u, u2);
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const C w = C(L::W[i]);
+        const C p = E.template p<i>();
+        eqv[i] = SH ? w * (drho + rho * p) : w * rho * (C(1) + p);
+        fneq[i] = f[i] - eqv[i];
+      });
+    }
+    // The alpha solve runs in FP64 whatever the compute type.
+    double fd[Q], nd[Q];
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      fd[i] = SH ? double(f[i]) + L::W[i] : double(f[i]);
+      nd[i] = double(fneq[i]);
+    });
+    const double guess = double(astate[cell]);
+    const double alpha = elbm_alpha<L, REF>(fd, nd, guess, branch, its);
+    astate[cell] = C(alpha);  // Algorithm.h:106
+    // tau = 1 / (alpha beta); collide with 1 / tau (Collision.h:240, 254).
+    const double tau = 1.0 / (alpha * a.beta);
+    const C rt = C(1.0 / tau);
+    C v[3] = {C(0), C(0), C(0)}, v2 = C(0);
+    if constexpr (FS == SRC_EDM) {
+      const C k = C(1.0) / rho;
+      for (int d = 0; d < L::dimD; ++d) v[d] = u[d] + F[d] * k;
+      v2 = v[0] * v[0];
+      for (int d = 1; d < L::dimD; ++d) v2 += v[d] * v[d];
+    }
+    C uF = u[0] * F[0];
+    for (int d = 1; d < L::dimD; ++d) uF += u[d] * F[d];
+    const RefEquilibrium<L, EQ, C> R2(rho, v, v2);
+    const FastEquilibrium<L, EQ, C> E2(v, v2);
+    // ELBM collideAndStream (Collision.h:243-258): feq := f - fneq.
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      C s = C(0);
+      if constexpr (REF) {
+        const C fe = f[i] - fneq[i];
+        if constexpr (FS == SRC_GUO) s = guo_ref<L, i>(gc, u, F);
+        if constexpr (FS == SRC_EDM) s = R2.template feq<i>() - fe;
+      } else {
+        const C w = C(L::W[i]);
+        if constexpr (FS == SRC_GUO) {
+          const C cF = cdot<L, i>(F);
+          const C cu = cdot<L, i>(u);
+          s = (C(3) * gc * w) * (cF - uF + C(3) * cu * cF);
+        }
+        if constexpr (FS == SRC_EDM) {
+          const C e2 = SH ? w * (drho + rho * E2.template p<i>())
+                          : w * rho * (C(1) + E2.template p<i>());
+          s = e2 - eqv[i];
+        }
+      }
+      const C out = f[i] - rt * fneq[i] + s;
+      store_pop<L, i, S, C>(dst + (size_t)i * a.N + cell, out);
+    });
+    if (a.store) store_fields<L, C>(a, cell, FSX, rho, u, F, alpha);
+  }
+  if (a.stats) {
+    __syncthreads();
+    if (branch >= 0) {
+      atomicAdd(&s_cnt[branch], 1u);
+      if (its) atomicAdd(&s_cnt[4], (unsigned)its);
+    }
+    __syncthreads();
+    if (threadIdx.x < 6 && s_cnt[threadIdx.x])
+      atomicAdd(&a.stats[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+  }
+}
+
+}  // namespace lbm

@@ -1,0 +1,52 @@
+// instantiate.hpp -- instantiates and registers the fused step kernels for
+// one compute type.  Included by exactly one translation unit per compute
+// type, so each type can be built with its own floating-point contraction
+// mode (FP64: off, for reference-order parity; FP32: fused).
+#pragma once
+
+#include "fused_step.hpp"
+#include "registry.hpp"
+
+namespace lbm {
+
+template <class L, class S, class C, int FS, int EQ, bool SH, int CO>
+hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
+                       hipStream_t st) {
+  if constexpr (CO == (int)CollisionType::BGK)
+    hipLaunchKernelGGL((k_bgk<L, S, C, FS, EQ, SH, 256>), dim3(grid),
+                       dim3(block), 0, st, a, fsx);
+  else
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256>), dim3(grid),
+                       dim3(block), 0, st, a, fsx);
+  return hipGetLastError();
+}
+
+template <class L, class S, class C, bool SH>
+void register_combo(Registry& r, int storage, int compute) {
+  auto add_eq = [&](auto eqc) {
+    constexpr int EQ = decltype(eqc)::value;
+    auto add_fs = [&](auto fsc) {
+      constexpr int FS = decltype(fsc)::value;
+      r.add({(int)L::Type, (int)CollisionType::BGK, FS, EQ, storage, compute, SH},
+            &launch_step<L, S, C, FS, EQ, SH, (int)CollisionType::BGK>);
+      r.add({(int)L::Type, (int)CollisionType::ELBM, FS, EQ, storage, compute, SH},
+            &launch_step<L, S, C, FS, EQ, SH, (int)CollisionType::ELBM>);
+    };
+    add_fs(std::integral_constant<int, SRC_ZERO>{});
+    add_fs(std::integral_constant<int, SRC_GUO>{});
+    add_fs(std::integral_constant<int, SRC_EDM>{});
+  };
+  add_eq(std::integral_constant<int, (int)EquilibriumType::TruncationMa3>{});
+  // There is no Exact equilibrium for D3Q19 (Equilibrium.h:36-126).
+  if constexpr (L::Type != LatticeType::D3Q19)
+    add_eq(std::integral_constant<int, (int)EquilibriumType::Exact>{});
+}
+
+template <class S, class C, bool SH>
+void register_all_lattices(Registry& r, int storage, int compute) {
+  register_combo<Lattice<LatticeType::D2Q9>, S, C, SH>(r, storage, compute);
+  register_combo<Lattice<LatticeType::D3Q19>, S, C, SH>(r, storage, compute);
+  register_combo<Lattice<LatticeType::D3Q27>, S, C, SH>(r, storage, compute);
+}
+
+}  // namespace lbm

@@ -1,0 +1,44 @@
+// registry.hpp -- run-time selection of the compile-time specialised kernels.
+//
+// The reference picks its operators at compile time through global aliases
+// (Collision_, ForcingScheme_, Equilibrium_, L: Collision.h:913-914,
+// ForcingScheme.h:200, Equilibrium.h:330, Lattice.h:804).  Here every
+// supported combination is instantiated once and looked up by key, so one
+// library serves every configuration without recompiling.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+
+#include "step_kernels.hpp"
+
+namespace lbm {
+
+using StepFn = hipError_t (*)(const StepArgs&, int fsx, int block,
+                              int grid, hipStream_t);
+
+struct KernelKey {
+  int lattice, collision, src, eq, storage, compute, shifted;
+  int64_t code() const {
+    return ((((((int64_t)lattice * 4 + collision) * 8 + src) * 4 + eq) * 4 +
+             storage) * 4 + compute) * 2 + shifted;
+  }
+};
+
+struct Registry {
+  std::map<int64_t, StepFn> step;
+  void add(const KernelKey& k, StepFn f) { step[k.code()] = f; }
+  StepFn find(const KernelKey& k) const {
+    auto it = step.find(k.code());
+    return it == step.end() ? nullptr : it->second;
+  }
+};
+
+void register_step_f64(Registry& r);
+void register_step_f32(Registry& r);
+
+const Registry& registry();
+
+}  // namespace lbm

@@ -1,0 +1,14 @@
+// step_f64.hip -- FP64-compute instantiations.  Built with -ffp-contract=off:
+// with unshifted FP64 storage these kernels evaluate every operator in the
+// reference's operation order, separately rounded, for bitwise parity with
+// the FP64 oracle wherever no libm call is involved.
+#include "instantiate.hpp"
+
+namespace lbm {
+
+void register_step_f64(Registry& r) {
+  register_all_lattices<double, double, false>(r, TYPE_F64, TYPE_F64);
+  register_all_lattices<double, double, true>(r, TYPE_F64, TYPE_F64);
+}
+
+}  // namespace lbm

@@ -1,0 +1,211 @@
+// step_kernels.hpp -- the fused collide-and-stream-pull kernels.
+//
+// One kernel launch is one reference time step (Algorithm.h:326-358 with the
+// per-cell body of Algorithm.h:97-126): every thread owns one destination
+// cell, pulls its Q populations from the source lattice (periodic wrap in the
+// index math replaces the halo fill of Communication.h:134-180 and
+// Boundary.h:45-102), forms the moments, force, equilibrium, forcing source
+// and relaxation in registers, and writes Q populations to the destination
+// lattice (two-lattice ping-pong, Algorithm.h:336).
+//
+// Work decomposition (MI355X): a workgroup of BLOCK threads covers BLOCK
+// consecutive cells of one row along the contiguous axis (z in 3D, y in 2D),
+// so every population is read and written as whole wavefront-wide 256 B
+// (FP32) segments.  The +-1 pull shift along the contiguous axis makes those
+// loads straddle one extra 128 B line, which the neighbouring wavefront of
+// the same row reads anyway, so the line is served from L2 and HBM traffic
+// stays at the algorithmic 2 Q sizeof(storage) bytes per cell.  Tiles are
+// remapped so each of the 8 XCDs sweeps a contiguous slab of rows (its L2
+// then holds the lines neighbouring tiles share).  Destination stores are
+// non-temporal: they are never re-read within the step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "physics.hpp"
+
+namespace lbm {
+
+struct StepArgs {
+  const void* src;
+  void* dst;
+  int nx, ny, nz;       // nz == 1 in 2D
+  int nk;               // extent of the contiguous axis (nz in 3D, ny in 2D)
+  int tiles_per_row;    // ceil(nk / blockDim.x)
+  int N;                // cells
+  double omega_f;       // 1 - 2 beta            (Collision.h:145)
+  double omega_eq;      // 2 beta                (Collision.h:147)
+  double beta;          // 1 / (2 tau)           (Collision.h:122)
+  double guo_coef;      // 1 - 1 / (2 tau0)      (ForcingScheme.h:115)
+  const void* ftab[3];  // per-component force tables (compute type) or NULL
+  int fax[3];           // axis indexing table d: 0 x, 1 y, 2 z, 3 constant
+  int store;            // write output fields on this step
+  double* rho_out;
+  double* uh_out;       // [D][N]
+  double* alpha_out;
+  double* F_out;        // [D][N]
+  void* alpha_state;    // ELBM warm-start field (compute type)
+  unsigned long long* stats;  // 6 ELBM counters or NULL
+};
+
+// Storage / compute codes (values of LBM_F64 / LBM_F32 / LBM_F16).
+enum : int { TYPE_F64 = 0, TYPE_F32 = 1, TYPE_F16 = 2 };
+
+// Forcing-scheme selector of the kernels: None and Shan-Chen both have a
+// zero collision source (ForcingScheme.h:68-78, 141-151).
+enum : int { SRC_ZERO = 0, SRC_GUO = 2, SRC_EDM = 4 };
+
+__device__ __forceinline__ int xcd_tile(int b, int G) {
+  // Workgroups are handed to the 8 XCDs round-robin (XCD = b mod 8).
+  if ((G & 7) != 0) return b;
+  return (b & 7) * (G >> 3) + (b >> 3);
+}
+
+template <class S>
+__device__ __forceinline__ S ld(const S* p) {
+  return *p;
+}
+
+template <class L, int i, class S, class C>
+__device__ __forceinline__ C to_compute(S v) {
+  if constexpr (std::is_same_v<S, _Float16>)
+    return C(float(v)) * C(1.0 / half_scale<L, i>());
+  else
+    return C(v);
+}
+
+template <class L, int i, class S, class C>
+__device__ __forceinline__ void store_pop(S* p, C v) {
+  if constexpr (std::is_same_v<S, _Float16>) {
+    const _Float16 h = _Float16(float(v * C(half_scale<L, i>())));
+    __builtin_nontemporal_store(h, p);
+  } else {
+    __builtin_nontemporal_store(S(v), p);
+  }
+}
+
+// Coordinates of the thread's cell and its periodic neighbours.
+struct CellPos {
+  int x, y, z, xm, xp, ym, yp, zm, zp;
+  bool active;
+};
+
+template <class L, int BLOCK>
+__device__ __forceinline__ CellPos cell_of_thread(const StepArgs& a) {
+  CellPos c;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int row = tile / a.tiles_per_row;
+  const int k = (tile - row * a.tiles_per_row) * (int)blockDim.x + threadIdx.x;
+  c.active = k < a.nk;
+  if constexpr (L::dimD == 3) {
+    c.x = row / a.ny;
+    c.y = row - c.x * a.ny;
+    c.z = c.active ? k : 0;
+  } else {
+    c.x = row;
+    c.y = c.active ? k : 0;
+    c.z = 0;
+  }
+  c.xm = c.x == 0 ? a.nx - 1 : c.x - 1;
+  c.xp = c.x == a.nx - 1 ? 0 : c.x + 1;
+  c.ym = c.y == 0 ? a.ny - 1 : c.y - 1;
+  c.yp = c.y == a.ny - 1 ? 0 : c.y + 1;
+  c.zm = c.z == 0 ? a.nz - 1 : c.z - 1;
+  c.zp = c.z == a.nz - 1 ? 0 : c.z + 1;
+  return c;
+}
+
+// Cell index of the pull source x - c_i (Collision.h:146), periodic.
+template <class L, int i>
+__device__ __forceinline__ int pull_index(const StepArgs& a, const CellPos& c) {
+  constexpr int cx = cel<L, i, 0>(), cy = cel<L, i, 1>(), cz = cel<L, i, 2>();
+  const int xs = cx > 0 ? c.xm : (cx < 0 ? c.xp : c.x);
+  const int ys = cy > 0 ? c.ym : (cy < 0 ? c.yp : c.y);
+  if constexpr (L::dimD == 3) {
+    const int zs = cz > 0 ? c.zm : (cz < 0 ? c.zp : c.z);
+    return (xs * a.ny + ys) * a.nz + zs;
+  } else {
+    return xs * a.ny + ys;
+  }
+}
+
+template <class L>
+__device__ __forceinline__ int self_index(const StepArgs& a, const CellPos& c) {
+  if constexpr (L::dimD == 3) return (c.x * a.ny + c.y) * a.nz + c.z;
+  else return c.x * a.ny + c.y;
+}
+
+// Force_::setForce from the host-built tables (Force.h:104-292).
+template <class L, class C>
+__device__ __forceinline__ void force_of_cell(const StepArgs& a,
+                                              const CellPos& c, C* F) {
+  const int pos[4] = {c.x, c.y, c.z, 0};
+  F[0] = F[1] = F[2] = C(0);
+#pragma unroll
+  for (int d = 0; d < L::dimD; ++d) {
+    const C* t = static_cast<const C*>(a.ftab[d]);
+    if (t) F[d] = t[pos[a.fax[d]]];
+  }
+}
+
+// Pull all Q populations of the cell into registers.
+template <class L, class S, class C>
+__device__ __forceinline__ void pull(const StepArgs& a, const CellPos& c,
+                                     C* f) {
+  const S* __restrict__ src = static_cast<const S*>(a.src);
+  static_for<L::dimQ>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    const S* base = src + (size_t)i * (size_t)a.N;
+    f[i] = to_compute<L, i, S, C>(ld(base + pull_index<L, i>(a, c)));
+  });
+}
+
+// ---- reference-order operators (FP64, unshifted) --------------------------
+
+// Guo source, reference order (ForcingScheme.h:99-117), raw velocity.
+template <class L, int i, class T>
+__device__ __forceinline__ T guo_ref(T gcoef, const T* u, const T* F) {
+  const T cu = cdot<L, i>(u);
+  const T s3 = T(3.0) * cu;
+  T r = ((T(cel<L, i, 0>()) - u[0]) + T(cel<L, i, 0>()) * s3) * F[0];
+  r += ((T(cel<L, i, 1>()) - u[1]) + T(cel<L, i, 1>()) * s3) * F[1];
+  if constexpr (L::dimD == 3)
+    r += ((T(cel<L, i, 2>()) - u[2]) + T(cel<L, i, 2>()) * s3) * F[2];
+  return gcoef * T(L::W[i]) * T(3.0) * r;
+}
+
+template <class L, int EQ, class T>
+struct RefEquilibrium {
+  // EQ == Exact keeps the per-cell factors; Ma3 is evaluated directly.
+  ExactFactors<L, T> ex;
+  T rho, u2;
+  const T* u;
+  __device__ __forceinline__ RefEquilibrium(T rho_, const T* u_, T u2_)
+      : ex(u_), rho(rho_), u2(u2_), u(u_) {}
+  template <int i>
+  __device__ __forceinline__ T feq() const {
+    if constexpr (EQ == (int)EquilibriumType::Exact) return ex.template feq<i>(rho);
+    else return feq_ma3_ref<L, i>(rho, u, u2);
+  }
+};
+
+// ---- shifted / fast operators ----------------------------------------------
+
+template <class L, int EQ, class T>
+struct FastEquilibrium {
+  // p_i = P_i(u) - 1 for either equilibrium.
+  Ma3Coefs<T> ma3;
+  ExactShifted<L, T> ex;
+  const T* u;
+  __device__ __forceinline__ FastEquilibrium(const T* u_, T u2)
+      : ma3(u2), ex(u_), u(u_) {}
+  template <int i>
+  __device__ __forceinline__ T p() const {
+    if constexpr (EQ == (int)EquilibriumType::Exact) return ex.template p<i>();
+    else return ma3.p(cdot<L, i>(u));
+  }
+};
+
+}  // namespace lbm

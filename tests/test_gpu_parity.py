@@ -1,0 +1,208 @@
+"""GPU parity: the HIP library through its C ABI against the FP64 oracle.
+
+Bar: FP64 compute with unshifted FP64 storage matches the oracle's rho, u and
+f to <= 1e-12 relative after N steps (in practice bitwise for BGK: both sides
+evaluate the reference's operation order without FMA contraction and the
+forces come from the same host sin); every reduced-precision configuration
+states its error against the oracle with the tolerance written in the test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from metalbm_amd import (Algorithm, CollisionType, Config, EquilibriumType, ForceType,
+                         ForcingSchemeType, LatticeType, Precision)
+
+pytestmark = pytest.mark.gpu
+
+D2Q9, D3Q19, D3Q27 = LatticeType.D2Q9, LatticeType.D3Q19, LatticeType.D3Q27
+SHAPES = {D2Q9: (20, 12, 1), D3Q19: (7, 6, 9), D3Q27: (5, 6, 7)}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_cfg(o, cfg: Config):
+    return o.Config(lattice=int(cfg.lattice), collision=int(cfg.collision),
+                    forcing_scheme=int(cfg.forcing_scheme), force=int(cfg.force),
+                    equilibrium=int(cfg.equilibrium), nx=cfg.nx, ny=cfg.ny, nz=cfg.nz,
+                    tau=cfg.tau, amplitude=cfg.amplitude, wavelength=cfg.wavelength)
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def run_both(o, cfg: Config, steps: int, amp: float = 1e-3, seed: int = 42):
+    oc = oracle_cfg(o, cfg)
+    u0 = o.noise_velocity(oc, seed, amp)
+    f0 = o.init_equilibrium(oc, np.ones(oc.cells), u0)
+    ref = o.run(oc, f0, steps)
+    with Algorithm(cfg) as a:
+        a.set_populations(f0)
+        a.iterate(steps, store=True)
+        f = a.populations()
+        rho, uh, al, F = a.store_fields()
+        stats = a.elbm_stats() if cfg.collision == CollisionType.ELBM else None
+    return ref, (f, rho, uh, al, F, stats)
+
+
+CASES = []
+for lat in (D2Q9, D3Q19, D3Q27):
+    for fs in ForcingSchemeType:
+        for force in (ForceType.None_, ForceType.Constant, ForceType.Sinusoidal,
+                      ForceType.Kolmogorov):
+            eqs = [EquilibriumType.TruncationMa3]
+            if lat != D3Q19 and fs == ForcingSchemeType.Guo:
+                eqs.append(EquilibriumType.Exact)
+            for eq in eqs:
+                CASES.append((lat, fs, force, eq))
+
+
+@pytest.mark.parametrize("lat,fs,force,eq", CASES,
+                         ids=[f"{l.name}-{s.name}-{f.name}-{e.name}" for l, s, f, e in CASES])
+def test_bgk_fp64_parity(oracle, lat, fs, force, eq):
+    nx, ny, nz = SHAPES[lat]
+    cfg = Config(lattice=lat, forcing_scheme=fs, force=force, equilibrium=eq, nx=nx, ny=ny,
+                 nz=nz, tau=0.6, amplitude=(1e-3, -2e-3, 1.5e-3), wavelength=(5.0, 4.0, 3.0))
+    say(f"case bgk-f64 {lat.name} {fs.name} {force.name} {eq.name} {nx}x{ny}x{nz}")
+    ref, (f, rho, uh, _, F, _) = run_both(oracle, cfg, 20, amp=2e-2)
+    assert rel_err(f, ref.f) <= 1e-12
+    assert rel_err(rho, ref.rho) <= 1e-12
+    assert rel_err(uh, ref.u) <= 1e-12
+    assert rel_err(F, ref.force) <= 1e-12 or np.all(ref.force == 0)
+    np.testing.assert_array_equal(F, ref.force)
+
+
+def test_c1_reference_config_1000_steps(oracle):
+    """D2Q9 SRT BGK 128x128, FP64, tau 0.6, Guo with Kolmogorov
+    F_x = 1e-5 sin(2 pi 4 y / 128), rho0 = 1, u0 seeded noise +-1e-3, 1000 steps."""
+    cfg = Config(lattice=D2Q9, forcing_scheme=ForcingSchemeType.Guo,
+                 force=ForceType.Kolmogorov, nx=128, ny=128, tau=0.6,
+                 amplitude=(1e-5, 0, 0), wavelength=(32.0, 1, 1))
+    say("case c1 d2q9 128x128 1000 steps")
+    ref, (f, rho, uh, _, _, _) = run_both(oracle, cfg, 1000, amp=1e-3, seed=42)
+    ef, er, eu = rel_err(f, ref.f), rel_err(rho, ref.rho), rel_err(uh, ref.u)
+    say(f"c1 rel errors f={ef:.3e} rho={er:.3e} u={eu:.3e} bitwise={np.array_equal(f, ref.f)}")
+    assert max(ef, er, eu) <= 1e-12
+
+
+@pytest.mark.parametrize("lat", [D2Q9, D3Q19, D3Q27])
+@pytest.mark.parametrize("eq", [EquilibriumType.TruncationMa3, EquilibriumType.Exact])
+def test_init_kernels_match_oracle(oracle, lat, eq):
+    if lat == D3Q19 and eq == EquilibriumType.Exact:
+        pytest.skip("no Exact equilibrium for D3Q19")
+    nx, ny, nz = SHAPES[lat]
+    cfg = Config(lattice=lat, equilibrium=eq, nx=nx, ny=ny, nz=nz)
+    oc = oracle_cfg(oracle, cfg)
+    say(f"case init {lat.name} {eq.name}")
+    with Algorithm(cfg) as a:
+        a.init_noise(1.0, 1e-3, 42)
+        fn = a.populations()
+        a.init_taylor_green(1.0, 0.05)
+        ft = a.populations()
+        rng = np.random.default_rng(0)
+        rho = 1 + rng.uniform(-1e-2, 1e-2, oc.cells)
+        u = rng.uniform(-1e-2, 1e-2, (oc.d, oc.cells))
+        a.init_equilibrium(rho, u)
+        fe = a.populations()
+    np.testing.assert_array_equal(
+        fn, oracle.init_equilibrium(oc, np.ones(oc.cells), oracle.noise_velocity(oc, 42, 1e-3)))
+    np.testing.assert_array_equal(
+        ft, oracle.init_equilibrium(oc, np.ones(oc.cells), oracle.taylor_green(oc, 0.05)))
+    np.testing.assert_array_equal(fe, oracle.init_equilibrium(oc, rho, u))
+
+
+@pytest.mark.parametrize("lat", [D2Q9, D3Q19, D3Q27])
+def test_elbm_fp64_parity(oracle, lat):
+    nx, ny, nz = SHAPES[lat]
+    cfg = Config(lattice=lat, collision=CollisionType.ELBM, forcing_scheme=ForcingSchemeType.Guo,
+                 force=ForceType.Kolmogorov, nx=nx, ny=ny, nz=nz, tau=0.5003,
+                 amplitude=(1e-5, 0, 0), wavelength=(4.0, 1, 1))
+    say(f"case elbm-f64 {lat.name}")
+    ref, (f, rho, uh, al, _, st) = run_both(oracle, cfg, 10, amp=3e-2, seed=7)
+    assert ref.branch[2] > 0, "the case must exercise Newton"
+    say(f"oracle branches {ref.branch.tolist()} its {ref.newton_iterations}; gpu {st}")
+    assert rel_err(f, ref.f) <= 1e-12
+    assert rel_err(rho, ref.rho) <= 1e-12
+    assert rel_err(uh, ref.u) <= 1e-12
+    np.testing.assert_allclose(al, ref.alpha, rtol=1e-9)
+    assert [st["small"], st["capped"], st["newton"], st["fallback"]] == ref.branch.tolist()
+    assert st["newton_iterations"] == ref.newton_iterations
+
+
+@pytest.mark.parametrize("storage,tol_f,tol_u", [(Precision.F32, 2e-6, 2e-5),
+                                                  (Precision.F16, 2e-3, 2e-2),
+                                                  (Precision.F64, 2e-6, 2e-5)])
+@pytest.mark.parametrize("lat", [D2Q9, D3Q19, D3Q27])
+def test_reduced_precision_error_vs_oracle(oracle, lat, storage, tol_f, tol_u):
+    """FP32 compute, shifted storage: stated error bounds against the FP64
+    oracle after 50 steps (f relative to max f; u_h relative to max |u|)."""
+    nx, ny, nz = SHAPES[lat]
+    cfg = Config(lattice=lat, forcing_scheme=ForcingSchemeType.Guo, force=ForceType.Constant,
+                 nx=nx, ny=ny, nz=nz, tau=0.55, amplitude=(1e-6, 0, 0),
+                 storage=storage, compute=Precision.F32, shifted=True)
+    say(f"case reduced {lat.name} storage={storage.name}")
+    ref, (f, rho, uh, _, _, _) = run_both(oracle, cfg, 50, amp=1e-2)
+    ef, eu = rel_err(f, ref.f), rel_err(uh, ref.u)
+    say(f"reduced {lat.name} {storage.name}: f {ef:.2e} u {eu:.2e}")
+    assert ef <= tol_f and eu <= tol_u
+
+
+def test_unshifted_fp32(oracle):
+    cfg = Config(lattice=D3Q19, forcing_scheme=ForcingSchemeType.Guo, force=ForceType.Constant,
+                 nx=8, ny=8, nz=8, tau=0.55, amplitude=(1e-6, 0, 0),
+                 storage=Precision.F32, compute=Precision.F32, shifted=False)
+    say("case unshifted fp32")
+    ref, (f, rho, uh, _, _, _) = run_both(oracle, cfg, 50, amp=1e-2)
+    assert rel_err(f, ref.f) <= 1e-5 and rel_err(uh, ref.u) <= 1e-3
+
+
+@pytest.mark.parametrize("lat", [D2Q9, D3Q27])
+def test_reduce_matches_oracle(oracle, lat):
+    nx, ny, nz = SHAPES[lat]
+    cfg = Config(lattice=lat, nx=nx * 3, ny=ny * 2, nz=nz, force=ForceType.Constant,
+                 amplitude=(1e-5, 2e-5, 0))
+    oc = oracle_cfg(oracle, cfg)
+    say(f"case reduce {lat.name}")
+    with Algorithm(cfg) as a:
+        a.init_taylor_green(1.0, 0.05)
+        a.iterate(3, store=True)
+        rho, uh, _, _ = a.store_fields()
+        m, e, w = a.reduce()
+    mo, eo, wo = oracle.reduce(oc, rho, uh)
+    assert abs(m - mo) <= 1e-12 * abs(mo)
+    assert abs(e - eo) <= 1e-12 * abs(eo)
+    assert abs(w - wo) <= 1e-12 * abs(wo)
+
+
+def test_c2_full_size_invariants():
+    """D3Q19 256^3, Guo with constant F = (1e-6, 0, 0), tau 0.55: size-
+    independent checks at the full headline size.  Mass is conserved to
+    round-off and the total hydrodynamic momentum grows by exactly
+    (1 - 1/(2 tau)) F N per step (raw-velocity Guo, ForcingScheme.h:99-117)."""
+    for storage, compute, shifted, tol in [(Precision.F64, Precision.F64, False, 1e-9),
+                                           (Precision.F32, Precision.F32, True, 2e-3)]:
+        cfg = Config(lattice=D3Q19, forcing_scheme=ForcingSchemeType.Guo,
+                     force=ForceType.Constant, nx=256, ny=256, nz=256, tau=0.55,
+                     amplitude=(1e-6, 0, 0), storage=storage, compute=compute,
+                     shifted=shifted)
+        say(f"case c2 256^3 storage={storage.name}")
+        N = cfg.cells
+        with Algorithm(cfg) as a:
+            a.init_noise(1.0, 1e-3, 42)
+            a.iterate(1, store=True)
+            rho1, u1, _, _ = a.store_fields()
+            m1 = a.reduce()[0]
+            a.iterate(4, store=True)
+            rho5, u5, _, _ = a.store_fields()
+            m5 = a.reduce()[0]
+        assert abs(m5 - m1) <= 1e-13 * N * (1 if storage == Precision.F64 else 1e4)
+        p1 = (rho1 * u1[0]).sum()
+        p5 = (rho5 * u5[0]).sum()
+        expect = 4 * (1 - 1 / (2 * 0.55)) * 1e-6 * N
+        say(f"c2 {storage.name}: momentum gain {p5 - p1:.6e} expect {expect:.6e}")
+        assert abs((p5 - p1) - expect) <= tol * expect

@@ -547,20 +547,35 @@ void ref_init_noise_velocity(const ref_config *cfg, uint64_t seed, double amp,
     }
 }
 
+/* sin and cos are evaluated in separate table loops: a compiler that sees
+ * sin(X) and cos(X) together may merge them into sincos(), which glibc does
+ * not round identically to sin() (e.g. X = 2 pi 19 / 20). */
+static void trig_table(int n, double *s, double *c) {
+  const double two_pi = 2.0 * M_PI;
+  for (int i = 0; i < n; ++i) s[i] = sin(two_pi * i / n);
+  for (int i = 0; i < n; ++i) c[i] = cos(two_pi * i / n);
+}
+
 void ref_init_taylor_green(const ref_config *cfg, double u0, double *u) {
   const int D = ref_lattice_d(cfg->lattice);
   const int64_t N = ref_cells(cfg);
-  const double two_pi = 2.0 * M_PI;
-  for (int x = 0; x < cfg->nx; ++x)
-    for (int y = 0; y < cfg->ny; ++y)
-      for (int z = 0; z < cfg->nz; ++z) {
-        int64_t c = ((int64_t)x * cfg->ny + y) * cfg->nz + z;
-        double X = two_pi * x / cfg->nx, Y = two_pi * y / cfg->ny;
-        double cz = D == 3 ? cos(two_pi * z / cfg->nz) : 1.0;
-        u[c] = u0 * sin(X) * cos(Y) * cz;
-        u[N + c] = -u0 * cos(X) * sin(Y) * cz;
+  const int nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+  double *t = (double *)malloc(sizeof(double) * 2 * (size_t)(nx + ny + nz));
+  double *sx = t, *cx = sx + nx, *sy = cx + nx, *cy = sy + ny, *sz = cy + ny,
+         *cz = sz + nz;
+  trig_table(nx, sx, cx);
+  trig_table(ny, sy, cy);
+  trig_table(nz, sz, cz);
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        int64_t c = ((int64_t)x * ny + y) * nz + z;
+        double fz = D == 3 ? cz[z] : 1.0;
+        u[c] = u0 * sx[x] * cy[y] * fz;
+        u[N + c] = -u0 * cx[x] * sy[y] * fz;
         if (D == 3) u[2 * N + c] = 0.0;
       }
+  free(t);
 }
 
 /* ------------------------------------------------------------------------ */

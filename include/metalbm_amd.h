@@ -60,6 +60,15 @@ enum { LBM_EQ_EXACT = 1, LBM_EQ_MA3 = 3 };                         /* Equilibriu
 /* Arithmetic types (storage of populations, compute of the step). */
 enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16 = 2 };
 
+/* Kernel shape of the BGK step. */
+enum {
+  LBM_KERNEL_AUTO = 0,    /* vector kernel when the contiguous extent allows */
+  LBM_KERNEL_CELL = 1,    /* one cell per lane */
+  LBM_KERNEL_VECTOR = 2   /* V consecutive cells per lane, aligned vector
+                             accesses; needs nz (3D) / ny (2D) divisible by
+                             64 V for some supported V */
+};
+
 /* Entropic alpha solver (ELBM only). */
 enum {
   LBM_ALPHA_NEWTON = 0,   /* reference Newton: warm start from the stored
@@ -84,6 +93,7 @@ typedef struct {
   int compute;          /* LBM_F64 | LBM_F32 */
   int alpha_mode;       /* LBM_ALPHA_* (ELBM) */
   double residual_tol;  /* H-residual tolerance of LBM_ALPHA_LEARNED */
+  int kernel;           /* LBM_KERNEL_* */
 } lbm_config;
 
 typedef struct lbm_handle lbm_handle;

@@ -29,7 +29,7 @@ from .capi import LBMError, LbmConfigC, LbmElbmStatsC, check, lib
 
 __all__ = [
     "LatticeType", "CollisionType", "ForcingSchemeType", "ForceType",
-    "EquilibriumType", "Precision", "AlphaMode", "Config", "Algorithm",
+    "EquilibriumType", "Precision", "AlphaMode", "KernelShape", "Config", "Algorithm",
     "LBMError", "lattice_q", "lattice_d",
 ]
 
@@ -70,6 +70,12 @@ class Precision(enum.IntEnum):
     F16 = 2
 
 
+class KernelShape(enum.IntEnum):
+    AUTO = 0
+    CELL = 1
+    VECTOR = 2
+
+
 class AlphaMode(enum.IntEnum):
     NEWTON = 0
     LEARNED = 1
@@ -104,12 +110,13 @@ class Config:
     compute: int = Precision.F64
     alpha_mode: int = AlphaMode.NEWTON
     residual_tol: float = 1e-10
+    kernel: int = KernelShape.AUTO
 
     def to_c(self) -> LbmConfigC:
         c = LbmConfigC()
         for name in ("lattice", "collision", "forcing_scheme", "force",
                      "equilibrium", "nx", "ny", "nz", "storage", "compute",
-                     "alpha_mode"):
+                     "alpha_mode", "kernel"):
             setattr(c, name, int(getattr(self, name)))
         c.shifted = 1 if self.shifted else 0
         c.tau = float(self.tau)

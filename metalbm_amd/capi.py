@@ -34,7 +34,7 @@ class LbmConfigC(ctypes.Structure):
         ("amplitude", ctypes.c_double * 3), ("wavelength", ctypes.c_double * 3),
         ("storage", ctypes.c_int), ("shifted", ctypes.c_int),
         ("compute", ctypes.c_int), ("alpha_mode", ctypes.c_int),
-        ("residual_tol", ctypes.c_double),
+        ("residual_tol", ctypes.c_double), ("kernel", ctypes.c_int),
     ]
 
 

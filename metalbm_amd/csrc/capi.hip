@@ -34,6 +34,7 @@ struct lbm_handle {
   lbm_config cfg{};
   int Q = 0, D = 0, N = 0;
   int nk = 0, rows = 0, block = 256, tiles_per_row = 1;
+  int vec = 0;              // cells per lane of the vector kernel (0: cell kernel)
   size_t elem = 8;          // bytes per stored population value
   int ctype = TYPE_F64;     // compute type
   void* f[2] = {nullptr, nullptr};
@@ -145,6 +146,14 @@ int validate(const lbm_config* c, std::string& why) {
   }
   if (c->compute == LBM_F32 && c->storage == LBM_F64 && !c->shifted) {
     why = "F32 compute with F64 storage requires shifted populations";
+    return LBM_E_INVALID;
+  }
+  if (c->kernel < LBM_KERNEL_AUTO || c->kernel > LBM_KERNEL_VECTOR) {
+    why = "kernel must be AUTO (0), CELL (1) or VECTOR (2)";
+    return LBM_E_INVALID;
+  }
+  if (c->kernel == LBM_KERNEL_VECTOR && c->collision != LBM_BGK) {
+    why = "the vector kernel implements BGK only";
     return LBM_E_INVALID;
   }
   if (c->collision == LBM_ELBM && c->alpha_mode != LBM_ALPHA_NEWTON &&
@@ -283,6 +292,7 @@ int lbm_config_default(lbm_config* c) {
   c->shifted = 0;
   c->alpha_mode = LBM_ALPHA_NEWTON;
   c->residual_tol = 1e-10;
+  c->kernel = LBM_KERNEL_AUTO;
   return LBM_OK;
 }
 
@@ -314,6 +324,22 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   KernelKey key{cfg->lattice, cfg->collision, src_of(cfg->forcing_scheme),
                 cfg->equilibrium, cfg->storage, h->ctype, cfg->shifted ? 1 : 0};
   h->step = registry().find(key);
+  if (cfg->collision == LBM_BGK && cfg->kernel != LBM_KERNEL_CELL) {
+    // Widest vector whose 64-lane segment tiles the contiguous axis.
+    static const int cands[3][3] = {{2, 0, 0}, {4, 2, 0}, {8, 4, 2}};
+    for (int V : cands[cfg->storage]) {
+      if (V == 0 || h->nk % (64 * V) != 0) continue;
+      KernelKey kv = key;
+      kv.vec = V;
+      if (StepFn fn = registry().find(kv)) {
+        h->step = fn;
+        h->vec = V;
+        h->block = 256;
+        h->tiles_per_row = h->nk / (64 * V);
+        break;
+      }
+    }
+  }
   auto bail = [&](int code) {
     g_create_error = h->err;
     release(h);
@@ -322,6 +348,11 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   };
   if (!h->step) {
     h->err = "no kernel instantiated for this storage/compute combination";
+    return bail(LBM_E_INVALID);
+  }
+  if (cfg->kernel == LBM_KERNEL_VECTOR && !h->vec) {
+    h->err = "the vector kernel needs nz (3D) or ny (2D) divisible by 64 V "
+             "(V = 2 for F64, 2 or 4 for F32, 2, 4 or 8 for F16)";
     return bail(LBM_E_INVALID);
   }
   const size_t bytes = (size_t)h->N * h->Q * h->elem;
@@ -496,7 +527,9 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
   if (store_last)
     if (int rc = ensure_fields(h)) return rc;
   StepArgs a = make_args(h);
-  const int grid = h->rows * h->tiles_per_row;
+  // Vector kernel: one wavefront per 64 V-cell segment, 4 per workgroup.
+  const int grid = h->vec ? (h->rows * h->tiles_per_row + 3) / 4
+                          : h->rows * h->tiles_per_row;
   for (int s = 0; s < nsteps; ++s) {
     a.src = h->f[h->cur];
     a.dst = h->f[1 - h->cur];

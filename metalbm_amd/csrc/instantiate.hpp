@@ -5,6 +5,7 @@
 #pragma once
 
 #include "fused_step.hpp"
+#include "fused_step_vec.hpp"
 #include "registry.hpp"
 
 namespace lbm {
@@ -21,16 +22,27 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
   return hipGetLastError();
 }
 
-template <class L, class S, class C, bool SH>
+template <class L, class S, class C, int FS, int EQ, bool SH, int V>
+hipError_t launch_step_vec(const StepArgs& a, int fsx, int block, int grid,
+                           hipStream_t st) {
+  (void)block;
+  hipLaunchKernelGGL((k_bgk_vec<L, S, C, FS, EQ, SH, V>), dim3(grid), dim3(256),
+                     0, st, a, fsx);
+  return hipGetLastError();
+}
+
+template <class L, class S, class C, bool SH, int... Vs>
 void register_combo(Registry& r, int storage, int compute) {
   auto add_eq = [&](auto eqc) {
     constexpr int EQ = decltype(eqc)::value;
     auto add_fs = [&](auto fsc) {
       constexpr int FS = decltype(fsc)::value;
-      r.add({(int)L::Type, (int)CollisionType::BGK, FS, EQ, storage, compute, SH},
+      r.add({(int)L::Type, (int)CollisionType::BGK, FS, EQ, storage, compute, SH, 0},
             &launch_step<L, S, C, FS, EQ, SH, (int)CollisionType::BGK>);
-      r.add({(int)L::Type, (int)CollisionType::ELBM, FS, EQ, storage, compute, SH},
+      r.add({(int)L::Type, (int)CollisionType::ELBM, FS, EQ, storage, compute, SH, 0},
             &launch_step<L, S, C, FS, EQ, SH, (int)CollisionType::ELBM>);
+      (r.add({(int)L::Type, (int)CollisionType::BGK, FS, EQ, storage, compute, SH, Vs},
+             &launch_step_vec<L, S, C, FS, EQ, SH, Vs>), ...);
     };
     add_fs(std::integral_constant<int, SRC_ZERO>{});
     add_fs(std::integral_constant<int, SRC_GUO>{});
@@ -42,11 +54,11 @@ void register_combo(Registry& r, int storage, int compute) {
     add_eq(std::integral_constant<int, (int)EquilibriumType::Exact>{});
 }
 
-template <class S, class C, bool SH>
+template <class S, class C, bool SH, int... Vs>
 void register_all_lattices(Registry& r, int storage, int compute) {
-  register_combo<Lattice<LatticeType::D2Q9>, S, C, SH>(r, storage, compute);
-  register_combo<Lattice<LatticeType::D3Q19>, S, C, SH>(r, storage, compute);
-  register_combo<Lattice<LatticeType::D3Q27>, S, C, SH>(r, storage, compute);
+  register_combo<Lattice<LatticeType::D2Q9>, S, C, SH, Vs...>(r, storage, compute);
+  register_combo<Lattice<LatticeType::D3Q19>, S, C, SH, Vs...>(r, storage, compute);
+  register_combo<Lattice<LatticeType::D3Q27>, S, C, SH, Vs...>(r, storage, compute);
 }
 
 }  // namespace lbm

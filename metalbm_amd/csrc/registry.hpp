@@ -19,11 +19,13 @@ namespace lbm {
 using StepFn = hipError_t (*)(const StepArgs&, int fsx, int block,
                               int grid, hipStream_t);
 
+// vec = 0 selects the one-cell-per-lane kernel; vec = V > 0 the kernel in
+// which each lane owns V consecutive cells (aligned vector accesses).
 struct KernelKey {
-  int lattice, collision, src, eq, storage, compute, shifted;
+  int lattice, collision, src, eq, storage, compute, shifted, vec;
   int64_t code() const {
-    return ((((((int64_t)lattice * 4 + collision) * 8 + src) * 4 + eq) * 4 +
-             storage) * 4 + compute) * 2 + shifted;
+    return (((((((int64_t)lattice * 4 + collision) * 8 + src) * 4 + eq) * 4 +
+              storage) * 4 + compute) * 2 + shifted) * 16 + vec;
   }
 };
 

@@ -5,10 +5,10 @@
 namespace lbm {
 
 void register_step_f32(Registry& r) {
-  register_all_lattices<float, float, false>(r, TYPE_F32, TYPE_F32);
-  register_all_lattices<float, float, true>(r, TYPE_F32, TYPE_F32);
-  register_all_lattices<_Float16, float, true>(r, TYPE_F16, TYPE_F32);
-  register_all_lattices<double, float, true>(r, TYPE_F64, TYPE_F32);
+  register_all_lattices<float, float, false, 4, 2>(r, TYPE_F32, TYPE_F32);
+  register_all_lattices<float, float, true, 4, 2>(r, TYPE_F32, TYPE_F32);
+  register_all_lattices<_Float16, float, true, 8, 4, 2>(r, TYPE_F16, TYPE_F32);
+  register_all_lattices<double, float, true, 2>(r, TYPE_F64, TYPE_F32);
 }
 
 }  // namespace lbm

@@ -7,8 +7,8 @@
 namespace lbm {
 
 void register_step_f64(Registry& r) {
-  register_all_lattices<double, double, false>(r, TYPE_F64, TYPE_F64);
-  register_all_lattices<double, double, true>(r, TYPE_F64, TYPE_F64);
+  register_all_lattices<double, double, false, 2>(r, TYPE_F64, TYPE_F64);
+  register_all_lattices<double, double, true, 2>(r, TYPE_F64, TYPE_F64);
 }
 
 }  // namespace lbm

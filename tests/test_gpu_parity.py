@@ -206,3 +206,57 @@ def test_c2_full_size_invariants():
         expect = 4 * (1 - 1 / (2 * 0.55)) * 1e-6 * N
         say(f"c2 {storage.name}: momentum gain {p5 - p1:.6e} expect {expect:.6e}")
         assert abs((p5 - p1) - expect) <= tol * expect
+
+
+# ---- vector kernel (V cells per lane, DPP wave rotate for the pull shift) ----
+
+VEC_CASES = [
+    # (lattice, shape, storage, compute, shifted)  -- segs = nk / (64 V)
+    (D3Q19, (3, 2, 128), Precision.F64, Precision.F64, False),   # V=2, one segment
+    (D3Q19, (2, 3, 256), Precision.F64, Precision.F64, False),   # V=2, two segments
+    (D3Q27, (2, 2, 384), Precision.F64, Precision.F64, False),   # V=2, three segments
+    (D2Q9, (5, 512, 1), Precision.F64, Precision.F64, False),    # 2D, four segments
+    (D3Q19, (2, 3, 256), Precision.F32, Precision.F32, True),    # V=4, one segment
+    (D3Q27, (2, 2, 512), Precision.F32, Precision.F32, True),    # V=4, two segments
+    (D3Q19, (2, 2, 384), Precision.F32, Precision.F32, True),    # V=2, three segments
+    (D2Q9, (4, 1024, 1), Precision.F32, Precision.F32, False),   # 2D unshifted, V=4
+    (D3Q27, (2, 2, 512), Precision.F16, Precision.F32, True),    # V=8, one segment
+    (D3Q19, (2, 2, 768), Precision.F16, Precision.F32, True),    # V=4, three segments
+    (D3Q27, (2, 2, 384), Precision.F16, Precision.F32, True),    # V=2, three segments
+    (D3Q19, (2, 3, 256), Precision.F64, Precision.F32, True),    # FP64 storage, V=2
+]
+
+
+@pytest.mark.parametrize("lat,shape,storage,compute,shifted", VEC_CASES,
+                         ids=[f"{c[0].name}-{'x'.join(map(str, c[1]))}-{c[2].name}"
+                              f"{'s' if c[4] else ''}-{c[3].name}" for c in VEC_CASES])
+@pytest.mark.parametrize("fs,force", [(ForcingSchemeType.Guo, ForceType.Sinusoidal),
+                                      (ForcingSchemeType.ExactDifferenceMethod,
+                                       ForceType.Kolmogorov)])
+def test_vector_kernel(oracle, lat, shape, storage, compute, shifted, fs, force):
+    nx, ny, nz = shape
+    base = dict(lattice=lat, forcing_scheme=fs, force=force, nx=nx, ny=ny, nz=nz, tau=0.6,
+                amplitude=(1e-4, -2e-4, 1e-4), wavelength=(7.0, 16.0, 32.0),
+                storage=storage, compute=compute, shifted=shifted)
+    say(f"case vector {lat.name} {shape} {storage.name}/{compute.name} shifted={shifted} {fs.name}")
+    vec = Config(kernel=2, **base)
+    ref, (f, rho, uh, _, F, _) = run_both(oracle, vec, 12, amp=2e-2)
+    cell = Config(kernel=1, **base)
+    _, (fc, rhoc, uhc, _, _, _) = run_both(oracle, cell, 12, amp=2e-2)
+    if compute == Precision.F64 and not shifted:
+        np.testing.assert_array_equal(f, fc)
+        assert rel_err(f, ref.f) <= 1e-12 and rel_err(uh, ref.u) <= 1e-12
+        assert rel_err(rho, ref.rho) <= 1e-12
+    else:
+        tol = 2e-3 if storage == Precision.F16 else 2e-6
+        assert rel_err(f, ref.f) <= tol and rel_err(fc, ref.f) <= tol
+        assert rel_err(f, fc) <= tol
+        assert rel_err(uh, ref.u) <= (2e-2 if storage == Precision.F16 else 2e-4)
+    np.testing.assert_array_equal(F, ref.force.astype(np.float32).astype(np.float64)
+                                  if compute == Precision.F32 else ref.force)
+
+
+def test_vector_kernel_rejects_bad_extent():
+    from metalbm_amd import LBMError
+    with pytest.raises(LBMError, match="divisible by 64"):
+        Algorithm(Config(lattice=D3Q19, nx=2, ny=2, nz=100, kernel=2))

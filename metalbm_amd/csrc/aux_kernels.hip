@@ -31,7 +31,7 @@ __global__ void k_init_eq(S* f, const double* rho, double rho0,
     constexpr int i = decltype(ic)::value;
     const double fe = E.template feq<i>();
     double val = SH ? fe - L::W[i] : fe;
-    if constexpr (std::is_same_v<S, _Float16>) val *= half_scale<L, i>();
+    if constexpr (std::is_same_v<S, _Float16>) val *= kHalfScale<L, i>;
     put(f + (size_t)i * N + c, val);
   });
 }
@@ -204,7 +204,7 @@ void tables_of(double* w, double* scale) {
   static_for<L::dimQ>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     w[i] = L::W[i];
-    scale[i] = half_scale<L, i>();
+    scale[i] = kHalfScale<L, i>;
   });
 }
 

@@ -5,9 +5,9 @@
 // V * sizeof(S) = 8..16 byte vector accesses, and a 64-lane wavefront covers
 // a segment of 64 V cells of one row.  The +-1 pull shift along the
 // contiguous axis is done in registers: the one value a lane needs from its
-// neighbour lane is moved with a DPP full-wavefront rotate (wave_ror:1 /
-// wave_rol:1), so no load is misaligned.  When the segment is the whole row
-// the rotate already wraps periodically; otherwise lane 0 (or 63) takes the
+// neighbour lane is moved with a cross-lane permute (a full-wavefront
+// rotate), so no load is misaligned.  When the segment is the whole row the
+// rotate already wraps periodically; otherwise lane 0 (or 63) takes the
 // value across the segment boundary from one extra uniform load.
 //
 // Same arithmetic as k_bgk per cell (reference order for FP64 unshifted).
@@ -17,31 +17,32 @@
 
 namespace lbm {
 
-// Lane l receives lane (l - 1) mod 64's value (DPP wave_ror:1).
-__device__ __forceinline__ int dpp_from_prev(int v) {
-  return __builtin_amdgcn_mov_dpp(v, 0x13C, 0xF, 0xF, false);
-}
-// Lane l receives lane (l + 1) mod 64's value (DPP wave_rol:1).
-__device__ __forceinline__ int dpp_from_next(int v) {
-  return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false);
-}
+// Cross-lane move of the one value each lane needs from its neighbour:
+// ds_bpermute (LDS crossbar, no LDS allocation) with the source lane index
+// precomputed per lane.  DPP wave_ror/wave_rol were tried first and
+// rejected: the compiler emits them in place (vdst == vsrc), and a wave64
+// rotate across 16-lane rows then reads lanes an earlier pass has already
+// overwritten, which corrupted results at full size.
+struct LaneXfer {
+  int from_prev, from_next;  // byte addresses of lanes (l - 1) mod 64, (l + 1) mod 64
+  __device__ __forceinline__ explicit LaneXfer(int lane)
+      : from_prev(((lane + 63) & 63) << 2), from_next(((lane + 1) & 63) << 2) {}
+};
 
 template <bool PREV, class S>
-__device__ __forceinline__ S lane_shift(S v) {
+__device__ __forceinline__ S lane_shift(const LaneXfer& x, S v) {
+  const int addr = PREV ? x.from_prev : x.from_next;
   if constexpr (sizeof(S) == 4) {
-    const int r = PREV ? dpp_from_prev(__builtin_bit_cast(int, v))
-                       : dpp_from_next(__builtin_bit_cast(int, v));
-    return __builtin_bit_cast(S, r);
+    return __builtin_bit_cast(S, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
   } else if constexpr (sizeof(S) == 8) {
     const long long b = __builtin_bit_cast(long long, v);
-    const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
-    const int rlo = PREV ? dpp_from_prev(lo) : dpp_from_next(lo);
-    const int rhi = PREV ? dpp_from_prev(hi) : dpp_from_next(hi);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
     return __builtin_bit_cast(
-        S, (long long)(((unsigned long long)(unsigned)rhi << 32) | (unsigned)rlo));
+        S, (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
   } else {
-    const int b = (int)__builtin_bit_cast(unsigned short, v);
-    const int r = PREV ? dpp_from_prev(b) : dpp_from_next(b);
+    const int r = __builtin_amdgcn_ds_bpermute(
+        addr, (int)__builtin_bit_cast(unsigned short, v));
     return __builtin_bit_cast(S, (unsigned short)r);
   }
 }
@@ -53,16 +54,62 @@ struct VecT {
 
 struct WavePos {
   int x, y, xm, xp, ym, yp;  // row coordinates (3D: x, y; 2D: x only)
+  LaneXfer xfer{0};
   int k0;                    // first contiguous-axis cell of this lane
   int seg_lo, seg_hi;        // segment [seg_lo, seg_hi) along the contiguous axis
   int lane;
   bool whole_row;
 };
 
+// Buffer-resource access (MUBUF): one SGPR resource per population lattice
+// row block, the row offset in soffset (uniform) and a single 32-bit lane
+// offset shared by all Q loads and stores, so no per-lane 64-bit address
+// arithmetic is needed.  Accesses beyond num_records return 0 / are dropped.
+constexpr int kBufferDword3 = 0x00020000;  // raw buffer, 32-bit data format
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base,
+                                                          unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                           (int)bytes, kBufferDword3);
+}
+
+template <int BYTES>
+struct BufIO;
+template <>
+struct BufIO<16> {
+  typedef unsigned raw __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0);
+  }
+  static __device__ __forceinline__ void store(raw v, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, vo, so, 2 /* nt */);
+  }
+};
+template <>
+struct BufIO<8> {
+  typedef unsigned raw __attribute__((ext_vector_type(2)));
+  static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+    return __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
+  }
+  static __device__ __forceinline__ void store(raw v, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, vo, so, 2);
+  }
+};
+template <>
+struct BufIO<4> {
+  typedef unsigned raw;
+  static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
+  }
+  static __device__ __forceinline__ void store(raw v, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, vo, so, 2);
+  }
+};
+
 // Pull population i for the lane's V cells (Collision.h:146, periodic).
 template <class L, int i, class S, class C, int V>
 __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
-                                         C (&f)[V]) {
+                                         unsigned voff, C (&f)[V]) {
   constexpr int D = L::dimD;
   constexpr int cx = cel<L, i, 0>(), cy = cel<L, i, 1>(), cz = cel<L, i, 2>();
   constexpr int ck = D == 3 ? cz : cy;  // shift along the contiguous axis
@@ -74,10 +121,11 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
   } else {
     row_off = xs * a.ny;
   }
-  const S* __restrict__ rowp =
-      static_cast<const S*>(a.src) + (size_t)i * (size_t)a.N + row_off;
+  const S* pop = static_cast<const S*>(a.src) + (size_t)i * (size_t)a.N;
+  const __amdgpu_buffer_rsrc_t r = make_rsrc(pop, (unsigned)a.N * (unsigned)sizeof(S));
   using vt = typename VecT<S, V>::type;
-  const vt v = *reinterpret_cast<const vt*>(rowp + w.k0);
+  using IO = BufIO<V * (int)sizeof(S)>;
+  const vt v = __builtin_bit_cast(vt, IO::load(r, voff, row_off * (int)sizeof(S)));
   S s[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) s[j] = v[j];
@@ -87,10 +135,10 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
     for (int j = 0; j < V; ++j) out[j] = s[j];
   } else if constexpr (ck > 0) {
     // cell k pulls k - 1
-    S edge = lane_shift<true>(s[V - 1]);
+    S edge = lane_shift<true>(w.xfer, s[V - 1]);
     if (!w.whole_row) {
       const int kb = w.seg_lo == 0 ? a.nk - 1 : w.seg_lo - 1;
-      const S b = rowp[kb];
+      const S b = pop[(unsigned)(row_off + kb)];
       if (w.lane == 0) edge = b;
     }
     out[0] = edge;
@@ -98,10 +146,10 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
     for (int j = 1; j < V; ++j) out[j] = s[j - 1];
   } else {
     // cell k pulls k + 1
-    S edge = lane_shift<false>(s[0]);
+    S edge = lane_shift<false>(w.xfer, s[0]);
     if (!w.whole_row) {
       const int kb = w.seg_hi == a.nk ? 0 : w.seg_hi;
-      const S b = rowp[kb];
+      const S b = pop[(unsigned)(row_off + kb)];
       if (w.lane == 63) edge = b;
     }
 #pragma unroll
@@ -113,30 +161,44 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
 }
 
 template <class L, int i, class S, class C, int V>
-__device__ __forceinline__ void store_vec(S* p, const C (&v)[V]) {
+__device__ __forceinline__ void store_vec(const StepArgs& a, int self_row,
+                                          unsigned voff, const C (&v)[V]) {
   using vt = typename VecT<S, V>::type;
+  using IO = BufIO<V * (int)sizeof(S)>;
   vt o;
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     if constexpr (std::is_same_v<S, _Float16>)
-      o[j] = _Float16(float(v[j] * C(half_scale<L, i>())));
+      o[j] = _Float16(float(v[j] * C(kHalfScale<L, i>)));
     else
       o[j] = S(v[j]);
   }
-  __builtin_nontemporal_store(o, reinterpret_cast<vt*>(p));
+  S* pop = static_cast<S*>(a.dst) + (size_t)i * (size_t)a.N;
+  // The row offset rides in the lane offset and soffset is the constant 0.
+  // With a register soffset the compiler assumes a >8-byte store cannot be
+  // corrupted by a VALU write to its data VGPRs right after issue, and emits
+  // none of the wait states it otherwise inserts; on gfx950 that overwrote
+  // store data at full size.
+  IO::store(__builtin_bit_cast(typename IO::raw, o),
+            make_rsrc(pop, (unsigned)a.N * (unsigned)sizeof(S)),
+            voff + (unsigned)self_row * (unsigned)sizeof(S), 0);
 }
 
 template <class L, class S, class C, int FS, int EQ, bool SH, int V>
 __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
   constexpr int Q = L::dimQ, D = L::dimD;
   constexpr bool REF = std::is_same_v<C, double> && !SH;
-  const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 + (int)(threadIdx.x >> 6);
+  // threadIdx.x >> 6 is the wavefront index: uniform, which readfirstlane
+  // makes visible to the compiler so all row addressing stays in SGPRs.
+  const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 +
+                   __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int segs = a.tiles_per_row;   // segments of 64 V cells per row
   const int row = wave / segs;
   const int rows = D == 3 ? a.nx * a.ny : a.nx;
   if (row >= rows) return;            // uniform per wavefront
   WavePos w;
   w.lane = threadIdx.x & 63;
+  w.xfer = LaneXfer(w.lane);
   const int seg = wave - row * segs;
   w.seg_lo = seg * 64 * V;
   w.seg_hi = w.seg_lo + 64 * V;
@@ -154,14 +216,14 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
   w.ym = w.y == 0 ? a.ny - 1 : w.y - 1;
   w.yp = w.y == a.ny - 1 ? 0 : w.y + 1;
 
+  const unsigned voff = (unsigned)w.k0 * (unsigned)sizeof(S);
   C f[Q][V];
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    pull_vec<L, i, S, C, V>(a, w, f[i]);
+    pull_vec<L, i, S, C, V>(a, w, voff, f[i]);
   });
 
   const int self_row = D == 3 ? (w.x * a.ny + w.y) * a.nz : w.x * a.ny;
-  S* __restrict__ dst = static_cast<S*>(a.dst) + self_row + w.k0;
   const C of = C(a.omega_f), oe = C(a.omega_eq), gc = C(a.guo_coef);
 
   // Per-cell moments and force.
@@ -209,7 +271,7 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
         }
         out[j] = of * f[i][j] + oe * fe + s;
       }
-      store_vec<L, i, S, C, V>(dst + (size_t)i * a.N, out);
+      store_vec<L, i, S, C, V>(a, self_row, voff, out);
     });
   } else {
     C uF[V];
@@ -244,7 +306,7 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
         }
         out[j] = of * f[i][j] + oe * eq + s;
       }
-      store_vec<L, i, S, C, V>(dst + (size_t)i * a.N, out);
+      store_vec<L, i, S, C, V>(a, self_row, voff, out);
     });
   }
   if (a.store) {

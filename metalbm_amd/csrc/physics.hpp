@@ -71,11 +71,21 @@ __host__ __device__ constexpr int first_nonzero() {
 // Power-of-two scale applied to shifted FP16 populations so that
 // (f_i - w_i) * s_i stays in FP16's normal range: s_i = 2^floor(log2(1/w_i)).
 template <class L, int i>
-__host__ __device__ constexpr double half_scale() {
+__host__ __device__ constexpr double half_scale_of() {
   double inv = 1.0 / L::W[i];
   double s = 1.0;
   while (s * 2.0 <= inv) s *= 2.0;
   return s;
+}
+// Constant-expression instances: the scale and its inverse are immediates in
+// the kernels (a plain constexpr call may be evaluated at run time).
+template <class L, int i>
+inline constexpr double kHalfScale = half_scale_of<L, i>();
+template <class L, int i>
+inline constexpr double kHalfScaleInv = 1.0 / half_scale_of<L, i>();
+template <class L, int i>
+__host__ __device__ constexpr double half_scale() {
+  return kHalfScale<L, i>;
 }
 
 // ---- Moment (Moment.h:13-47) -------------------------------------------------

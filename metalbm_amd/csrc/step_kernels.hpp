@@ -71,7 +71,7 @@ __device__ __forceinline__ S ld(const S* p) {
 template <class L, int i, class S, class C>
 __device__ __forceinline__ C to_compute(S v) {
   if constexpr (std::is_same_v<S, _Float16>)
-    return C(float(v)) * C(1.0 / half_scale<L, i>());
+    return C(float(v)) * C(kHalfScaleInv<L, i>);
   else
     return C(v);
 }
@@ -79,7 +79,7 @@ __device__ __forceinline__ C to_compute(S v) {
 template <class L, int i, class S, class C>
 __device__ __forceinline__ void store_pop(S* p, C v) {
   if constexpr (std::is_same_v<S, _Float16>) {
-    const _Float16 h = _Float16(float(v * C(half_scale<L, i>())));
+    const _Float16 h = _Float16(float(v * C(kHalfScale<L, i>)));
     __builtin_nontemporal_store(h, p);
   } else {
     __builtin_nontemporal_store(S(v), p);

@@ -129,6 +129,29 @@ class Hip:
     def sync(self):
         self.ok(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
 
+    def copy_calibration(self, nbytes: int, reps: int) -> float:
+        """hipMemcpy device-to-device of nbytes, reps times (the runtime's own
+        copy kernel): a plain copy of known size for scaling rocprofv3's
+        FETCH_SIZE / WRITE_SIZE counters.  Returns GB/s."""
+        vp = ctypes.c_void_p
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+        self.L.hipFree.argtypes = [vp]
+        self.L.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+        self.L.hipMemset.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
+        a, b = vp(), vp()
+        self.ok(self.L.hipMalloc(ctypes.byref(a), nbytes), "hipMalloc")
+        self.ok(self.L.hipMalloc(ctypes.byref(b), nbytes), "hipMalloc")
+        self.ok(self.L.hipMemset(a, 0, nbytes), "hipMemset")
+        self.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            self.ok(self.L.hipMemcpy(b, a, nbytes, 3), "hipMemcpy")   # 3: device to device
+        self.sync()
+        t = time.perf_counter() - t0
+        self.L.hipFree(a)
+        self.L.hipFree(b)
+        return 2.0 * nbytes * reps / t / 1e9
+
 
 def make_config(spec):
     import metalbm_amd as m
@@ -214,7 +237,13 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 1 GiB (counter calibration)")
     args = ap.parse_args(argv)
+    if args.copy_calibration:
+        gbs = Hip().copy_calibration(1 << 30, 5)
+        print(json.dumps({"copy_bytes_per_launch": 1 << 30, "launches": 5, "GB/s": round(gbs, 1)}))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs one process-local domain "
               "on one MI355X (no halo exchange); run it with --gpus 1", file=sys.stderr)

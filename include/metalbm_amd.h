@@ -74,8 +74,13 @@ enum {
   LBM_ALPHA_NEWTON = 0,   /* reference Newton: warm start from the stored
                              alpha, |step| <= 1e-8, <= 50 iterations
                              (Collision.h:328-375, EntropicStep.h:111-140) */
-  LBM_ALPHA_LEARNED = 1   /* MLP guess + one Newton correction, certified by
-                             the H-residual bound; Newton fallback */
+  LBM_ALPHA_LEARNED = 1,  /* start from the MLP's alpha, then residual Newton:
+                             accept when |G| <= residual_tol or when the
+                             Taylor bound of the next Newton point is;
+                             alpha = 2 fallback as the reference.  No alpha
+                             state is read or written. */
+  LBM_ALPHA_WARM_RESIDUAL = 2  /* the same residual Newton started from the
+                             stored alpha: the method without the network */
 };
 
 typedef struct {
@@ -168,6 +173,47 @@ int lbm_reduce(lbm_handle *h, double *mass, double *energy, double *enstrophy);
 /* Entropic statistics accumulated since creation or the last reset. */
 int lbm_get_elbm_stats(lbm_handle *h, lbm_elbm_stats *out);
 int lbm_reset_elbm_stats(lbm_handle *h);
+
+/* ---- Learned alpha (ELBM on D2Q9 and D3Q27) ------------------------------
+ * No counterpart in the reference, which solves alpha by Newton only
+ * (Collision.h:328-349).  A ReLU MLP (D2Q9: 9-32-32-1, D3Q27: 27-64-64-1,
+ * FP32) maps a cell's normalised non-equilibrium moments to a first alpha.
+ * Parameter layout: W1[h1][nin] b1[h1] W2[h2][h1] b2[h2] W3[h2] b3.
+ * Features: m_k = (1/rho) sum_i fneq_i prod_d phi_{a_kd}(c_id) over the
+ * tensor basis {1, c, c^2}^D, k = digits (a_x a_y [a_z]) in base 3. */
+int lbm_mlp_shape(const lbm_handle *h, int *nin, int *h1, int *h2, int *nparams);
+/* Allocates the network and initialises it (He-uniform, SplitMix64 seed). */
+int lbm_mlp_init(lbm_handle *h, uint64_t seed);
+int lbm_mlp_set_params(lbm_handle *h, const float *params);
+int lbm_mlp_get_params(lbm_handle *h, float *params);
+/* z_k = (m_k - in_mean_k) in_scale_k; alpha = out_mean + out_scale y. */
+int lbm_mlp_set_normalization(lbm_handle *h, const float *in_mean,
+                              const float *in_scale, double out_mean,
+                              double out_scale);
+int lbm_mlp_get_normalization(lbm_handle *h, float *in_mean, float *in_scale,
+                              double *out_mean, double *out_scale);
+/* Gradient of L = 1/(2n) sum (y - t)^2, t = (alpha - out_mean)/out_scale,
+ * over the n host samples x[n][nin], computed by the training kernels. */
+int lbm_mlp_gradient(lbm_handle *h, const float *x, const double *alpha, int n,
+                     float *grad, double *loss);
+/* alpha_hat for n host feature rows, computed by the inference code. */
+int lbm_mlp_predict(lbm_handle *h, const float *x, int n, double *alpha);
+/* Logs (features, alpha) of a `fraction` of the cells Newton solves during
+ * subsequent steps, up to `capacity` samples, into device memory. */
+int lbm_sample_log_enable(lbm_handle *h, int64_t capacity, double fraction,
+                          uint32_t salt);
+int lbm_sample_log_disable(lbm_handle *h);
+int lbm_sample_log_count(lbm_handle *h, int64_t *n);
+int lbm_sample_log_copy(lbm_handle *h, float *x, double *alpha, int64_t n);
+/* Trains on the logged samples on the GPU, starting from the current
+ * parameters (lbm_mlp_init(seed) first if none were initialised or loaded):
+ * sets the normalisation from the samples, then `iterations` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on
+ * minibatches of `batch` drawn by a seeded shuffle.  Mean losses of the
+ * first and last step are returned (either pointer may be NULL). */
+int lbm_mlp_train(lbm_handle *h, int iterations, int batch, double lr,
+                  uint64_t seed, double *loss_first, double *loss_last);
+/* Switches the ELBM alpha solver (LBM_ALPHA_*) and its residual tolerance. */
+int lbm_set_alpha_mode(lbm_handle *h, int mode, double residual_tol);
 
 /* Stream control: the handle enqueues on `stream` (a hipStream_t, may be
  * NULL for the null stream) from now on. */

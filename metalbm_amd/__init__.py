@@ -77,8 +77,9 @@ class KernelShape(enum.IntEnum):
 
 
 class AlphaMode(enum.IntEnum):
-    NEWTON = 0
-    LEARNED = 1
+    NEWTON = 0          # reference Newton (Collision.h:328-375)
+    LEARNED = 1         # MLP start + residual Newton with Taylor certificate
+    WARM_RESIDUAL = 2   # the same residual Newton from the stored alpha
 
 
 _QD = {LatticeType.D2Q9: (9, 2), LatticeType.D3Q19: (19, 3), LatticeType.D3Q27: (27, 3)}
@@ -141,6 +142,10 @@ class Config:
 
 def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _f64(a, shape) -> np.ndarray:
@@ -257,3 +262,83 @@ class Algorithm:
     @property
     def steps_done(self) -> int:
         return int(lib().lbm_steps_done(self._h))
+
+    # -- learned alpha ---------------------------------------------------------
+    def mlp_shape(self):
+        """(nin, h1, h2, nparams) of the alpha predictor for this lattice."""
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(lib().lbm_mlp_shape(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def mlp_init(self, seed: int) -> None:
+        self._check(lib().lbm_mlp_init(self._h, seed))
+
+    def mlp_params(self) -> np.ndarray:
+        p = np.empty(self.mlp_shape()[3], dtype=np.float32)
+        self._check(lib().lbm_mlp_get_params(self._h, _fp(p)))
+        return p
+
+    def set_mlp_params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != self.mlp_shape()[3]:
+            raise LBMError(-1, "wrong parameter count")
+        self._check(lib().lbm_mlp_set_params(self._h, _fp(a)))
+
+    def set_normalization(self, in_mean, in_scale, out_mean: float, out_scale: float) -> None:
+        m = np.ascontiguousarray(in_mean, dtype=np.float32)
+        s = np.ascontiguousarray(in_scale, dtype=np.float32)
+        self._check(lib().lbm_mlp_set_normalization(self._h, _fp(m), _fp(s), out_mean, out_scale))
+
+    def normalization(self):
+        nin = self.mlp_shape()[0]
+        m = np.empty(nin, dtype=np.float32)
+        s = np.empty(nin, dtype=np.float32)
+        om, os_ = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lbm_mlp_get_normalization(self._h, _fp(m), _fp(s), ctypes.byref(om),
+                                                    ctypes.byref(os_)))
+        return m, s, om.value, os_.value
+
+    def mlp_gradient(self, x, alpha):
+        """(grad[nparams], loss) over samples x[n, nin] with targets alpha[n]."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        g = np.empty(self.mlp_shape()[3], dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(lib().lbm_mlp_gradient(self._h, _fp(x), _dp(a), int(a.size), _fp(g),
+                                           ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        n = x.shape[0]
+        out = np.empty(n)
+        self._check(lib().lbm_mlp_predict(self._h, _fp(x), n, _dp(out)))
+        return out
+
+    def sample_log_enable(self, capacity: int, fraction: float = 1.0, salt: int = 0) -> None:
+        self._check(lib().lbm_sample_log_enable(self._h, capacity, fraction, salt))
+
+    def sample_log_disable(self) -> None:
+        self._check(lib().lbm_sample_log_disable(self._h))
+
+    def sample_log(self):
+        """(features[n, nin], alpha[n]) logged so far."""
+        n = ctypes.c_int64()
+        self._check(lib().lbm_sample_log_count(self._h, ctypes.byref(n)))
+        nin = self.mlp_shape()[0]
+        x = np.empty((n.value, nin), dtype=np.float32)
+        a = np.empty(n.value)
+        if n.value:
+            self._check(lib().lbm_sample_log_copy(self._h, _fp(x), _dp(a), n.value))
+        return x, a
+
+    def mlp_train(self, iterations: int, batch: int, lr: float, seed: int = 0):
+        """Adam on the logged samples; returns (first loss, last loss)."""
+        l0, l1 = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lbm_mlp_train(self._h, iterations, batch, lr, seed, ctypes.byref(l0),
+                                        ctypes.byref(l1)))
+        return l0.value, l1.value
+
+    def set_alpha_mode(self, mode, residual_tol: float = 1e-10) -> None:
+        self._check(lib().lbm_set_alpha_mode(self._h, int(mode), residual_tol))
+

@@ -28,9 +28,14 @@ COMMON = ["-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", f"-I{INCLUDE}"]
 SOURCES = {
-    "step_f64.hip": ["-ffp-contract=off"],
-    "step_f32.hip": ["-ffp-contract=fast"],
+    "step_f32_d3q27.hip": ["-ffp-contract=fast"],
+    "step_f32_d3q19.hip": ["-ffp-contract=fast"],
+    "step_f32_d2q9.hip": ["-ffp-contract=fast"],
+    "step_f64_d3q27.hip": ["-ffp-contract=off"],
+    "step_f64_d3q19.hip": ["-ffp-contract=off"],
+    "step_f64_d2q9.hip": ["-ffp-contract=off"],
     "aux_kernels.hip": ["-ffp-contract=off"],
+    "mlp_train.hip": ["-ffp-contract=fast"],
     "capi.hip": ["-ffp-contract=off"],
 }
 
@@ -68,7 +73,8 @@ def _compile(src: str, flags, changed_only: bool) -> str:
 
 def build(changed_only: bool = False, verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+    workers = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], changed_only),
                            SOURCES.items()))
     tmp = LIB + ".tmp"

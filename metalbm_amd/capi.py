@@ -21,6 +21,10 @@ EXPORTS = (
     "lbm_set_alpha", "lbm_get_alpha", "lbm_step", "lbm_get_fields", "lbm_reduce",
     "lbm_get_elbm_stats", "lbm_reset_elbm_stats", "lbm_set_stream",
     "lbm_synchronize", "lbm_device_populations", "lbm_steps_done",
+    "lbm_mlp_shape", "lbm_mlp_init", "lbm_mlp_set_params", "lbm_mlp_get_params",
+    "lbm_mlp_set_normalization", "lbm_mlp_get_normalization", "lbm_mlp_gradient",
+    "lbm_mlp_predict", "lbm_sample_log_enable", "lbm_sample_log_disable",
+    "lbm_sample_log_count", "lbm_sample_log_copy", "lbm_mlp_train", "lbm_set_alpha_mode",
 )
 
 
@@ -94,6 +98,23 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(ctypes.c_int)]
     L.lbm_steps_done.argtypes = [H]
     L.lbm_steps_done.restype = ctypes.c_int64
+    ip = ctypes.POINTER(ctypes.c_int)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.lbm_mlp_shape.argtypes = [H, ip, ip, ip, ip]
+    L.lbm_mlp_init.argtypes = [H, ctypes.c_uint64]
+    L.lbm_mlp_set_params.argtypes = [H, fp]
+    L.lbm_mlp_get_params.argtypes = [H, fp]
+    L.lbm_mlp_set_normalization.argtypes = [H, fp, fp, ctypes.c_double, ctypes.c_double]
+    L.lbm_mlp_get_normalization.argtypes = [H, fp, fp, dp, dp]
+    L.lbm_mlp_gradient.argtypes = [H, fp, dp, ctypes.c_int, fp, dp]
+    L.lbm_mlp_predict.argtypes = [H, fp, ctypes.c_int, dp]
+    L.lbm_sample_log_enable.argtypes = [H, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32]
+    L.lbm_sample_log_disable.argtypes = [H]
+    L.lbm_sample_log_count.argtypes = [H, ctypes.POINTER(ctypes.c_int64)]
+    L.lbm_sample_log_copy.argtypes = [H, fp, dp, ctypes.c_int64]
+    L.lbm_mlp_train.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
+                                dp, dp]
+    L.lbm_set_alpha_mode.argtypes = [H, ctypes.c_int, ctypes.c_double]
     _lib = L
     return L
 

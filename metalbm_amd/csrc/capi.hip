@@ -6,6 +6,7 @@
 // Errors are returned as codes with a message; nothing exits.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +15,7 @@
 
 #include "../../include/metalbm_amd.h"
 #include "aux_kernels.hpp"
+#include "mlp_train.hpp"
 #include "registry.hpp"
 
 using namespace lbm;
@@ -22,8 +24,12 @@ namespace lbm {
 const Registry& registry() {
   static const Registry r = [] {
     Registry x;
-    register_step_f64(x);
-    register_step_f32(x);
+    register_step_f64_d2q9(x);
+    register_step_f64_d3q19(x);
+    register_step_f64_d3q27(x);
+    register_step_f32_d2q9(x);
+    register_step_f32_d3q19(x);
+    register_step_f32_d3q27(x);
     return x;
   }();
   return r;
@@ -53,6 +59,24 @@ struct lbm_handle {
   int64_t steps = 0;
   double w[27], scale[27];
   std::string err;
+  // entropic solver
+  int alpha_mode = LBM_ALPHA_NEWTON;
+  double residual_tol = 1e-10;
+  // learned alpha: parameter block (params | mean | scale | out_mean, out_scale)
+  int mlp_nin = 0, mlp_h1 = 0, mlp_h2 = 0, mlp_np = 0, mlp_nb = 0;
+  float* mlp = nullptr;
+  float* mlp_m = nullptr;   // Adam first moments
+  float* mlp_v = nullptr;   // Adam second moments
+  int adam_t = 0;
+  bool mlp_ready = false;   // normalisation set (trained or loaded)
+  bool mlp_has_params = false;  // initialised or loaded (else all zero)
+  // (features, alpha) sample log
+  float* log_x = nullptr;
+  double* log_alpha = nullptr;
+  unsigned long long* log_count = nullptr;
+  long long log_cap = 0;
+  unsigned log_threshold = 0, log_salt = 0;
+  bool log_on = false;
 };
 
 namespace {
@@ -157,8 +181,18 @@ int validate(const lbm_config* c, std::string& why) {
     return LBM_E_INVALID;
   }
   if (c->collision == LBM_ELBM && c->alpha_mode != LBM_ALPHA_NEWTON &&
-      c->alpha_mode != LBM_ALPHA_LEARNED) {
-    why = "alpha_mode must be NEWTON (0) or LEARNED (1)";
+      c->alpha_mode != LBM_ALPHA_LEARNED && c->alpha_mode != LBM_ALPHA_WARM_RESIDUAL) {
+    why = "alpha_mode must be NEWTON (0), LEARNED (1) or WARM_RESIDUAL (2)";
+    return LBM_E_INVALID;
+  }
+  if (c->collision == LBM_ELBM && c->alpha_mode == LBM_ALPHA_LEARNED &&
+      lat == LBM_D3Q19) {
+    why = "learned alpha is defined for D2Q9 and D3Q27 (tensor moment basis)";
+    return LBM_E_INVALID;
+  }
+  if (c->collision == LBM_ELBM && c->alpha_mode != LBM_ALPHA_NEWTON &&
+      !(c->residual_tol > 0.0)) {
+    why = "residual_tol must be positive";
     return LBM_E_INVALID;
   }
   return LBM_OK;
@@ -240,7 +274,9 @@ int ensure_fields(lbm_handle* h) {
 void release(lbm_handle* h) {
   for (void* p : {h->f[0], h->f[1], h->alpha, (void*)h->fields,
                   (void*)h->staging, (void*)h->reduce_buf, h->ftab[0],
-                  h->ftab[1], h->ftab[2], (void*)h->stats})
+                  h->ftab[1], h->ftab[2], (void*)h->stats, (void*)h->mlp,
+                  (void*)h->mlp_m, (void*)h->mlp_v, (void*)h->log_x,
+                  (void*)h->log_alpha, (void*)h->log_count})
     if (p) (void)hipFree(p);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
 }
@@ -265,6 +301,16 @@ StepArgs make_args(const lbm_handle* h) {
   }
   a.alpha_state = h->alpha;
   a.stats = h->stats;
+  a.alpha_mode = h->alpha_mode;
+  a.residual_tol = h->residual_tol;
+  a.mlp = h->mlp;
+  if (h->log_on) {
+    a.log_x = h->log_x;
+    a.log_alpha = h->log_alpha;
+    a.log_count = h->log_count;
+    a.log_cap = h->log_cap;
+    a.log_threshold = h->log_threshold;
+  }
   return a;
 }
 
@@ -305,11 +351,10 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   *out = nullptr;
   std::string why;
   if (validate(cfg, why) != LBM_OK) return fail(nullptr, LBM_E_INVALID, why);
-  if (cfg->collision == LBM_ELBM && cfg->alpha_mode == LBM_ALPHA_LEARNED)
-    return fail(nullptr, LBM_E_INVALID,
-                "LEARNED alpha needs a trained network (not yet loaded)");
   lbm_handle* h = new lbm_handle();
   h->cfg = *cfg;
+  h->alpha_mode = cfg->collision == LBM_ELBM ? cfg->alpha_mode : LBM_ALPHA_NEWTON;
+  h->residual_tol = cfg->residual_tol > 0.0 ? cfg->residual_tol : 1e-10;
   if (lattice_tables(cfg->lattice, h->w, h->scale, &h->Q, &h->D) != 0) {
     delete h;
     return fail(nullptr, LBM_E_INVALID, "unknown lattice");
@@ -526,6 +571,10 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
   if (nsteps == 0) return LBM_OK;
   if (store_last)
     if (int rc = ensure_fields(h)) return rc;
+  if (h->cfg.collision == LBM_ELBM && h->alpha_mode == LBM_ALPHA_LEARNED && !h->mlp_ready)
+    return fail(h, LBM_E_STATE,
+                "learned alpha needs a network: lbm_mlp_train or lbm_mlp_set_params + "
+                "lbm_mlp_set_normalization");
   StepArgs a = make_args(h);
   // Vector kernel: one wavefront per 64 V-cell segment, 4 per workgroup.
   const int grid = h->vec ? (h->rows * h->tiles_per_row + 3) / 4
@@ -534,6 +583,7 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
     a.src = h->f[h->cur];
     a.dst = h->f[1 - h->cur];
     a.store = (store_last && s == nsteps - 1) ? 1 : 0;
+    a.log_salt = h->log_salt + (unsigned)h->steps * 0x9E3779B9u;
     if (a.store) {
       a.rho_out = h->fields;
       a.uh_out = h->fields + h->N;
@@ -634,5 +684,359 @@ int lbm_device_populations(lbm_handle* h, void** ptr, int* elem_bytes) {
 }
 
 int64_t lbm_steps_done(const lbm_handle* h) { return h ? h->steps : -1; }
+
+// ---- learned alpha ----------------------------------------------------------
+
+namespace {
+
+int need_mlp_lattice(lbm_handle* h) {
+  if (!h) return LBM_E_INVALID;
+  if (h->cfg.collision != LBM_ELBM)
+    return fail(h, LBM_E_STATE, "learned alpha belongs to the ELBM collision");
+  if (h->mlp_nin == 0 &&
+      mlp_shape(h->cfg.lattice, &h->mlp_nin, &h->mlp_h1, &h->mlp_h2, &h->mlp_np,
+                &h->mlp_nb) != 0)
+    return fail(h, LBM_E_INVALID, "learned alpha is defined for D2Q9 and D3Q27");
+  return LBM_OK;
+}
+
+int ensure_mlp(lbm_handle* h) {
+  if (int rc = need_mlp_lattice(h)) return rc;
+  if (!h->mlp) {
+    HIP_TRY(h, hipMalloc(&h->mlp, sizeof(float) * h->mlp_nb));
+    HIP_TRY(h, hipMalloc(&h->mlp_m, sizeof(float) * h->mlp_np));
+    HIP_TRY(h, hipMalloc(&h->mlp_v, sizeof(float) * h->mlp_np));
+    HIP_TRY(h, hipMemset(h->mlp, 0, sizeof(float) * h->mlp_nb));
+    HIP_TRY(h, hipMemset(h->mlp_m, 0, sizeof(float) * h->mlp_np));
+    HIP_TRY(h, hipMemset(h->mlp_v, 0, sizeof(float) * h->mlp_np));
+  }
+  return LBM_OK;
+}
+
+uint64_t mix64(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+double unit01(uint64_t seed, uint64_t k) { return (double)(mix64(seed, k) >> 11) * 0x1.0p-53; }
+
+}  // namespace
+
+int lbm_mlp_shape(const lbm_handle* h, int* nin, int* h1, int* h2, int* nparams) {
+  if (!h) return LBM_E_INVALID;
+  int a, b, c, d, e;
+  if (mlp_shape(h->cfg.lattice, &a, &b, &c, &d, &e) != 0) return LBM_E_INVALID;
+  if (nin) *nin = a;
+  if (h1) *h1 = b;
+  if (h2) *h2 = c;
+  if (nparams) *nparams = d;
+  return LBM_OK;
+}
+
+int lbm_mlp_init(lbm_handle* h, uint64_t seed) {
+  if (int rc = ensure_mlp(h)) return rc;
+  std::vector<float> p(h->mlp_nb, 0.0f);
+  const int nin = h->mlp_nin, h1 = h->mlp_h1, h2 = h->mlp_h2;
+  uint64_t k = 0;
+  auto fill = [&](int off, int count, int fan_in) {
+    const double lim = std::sqrt(6.0 / fan_in);   // He-uniform for ReLU
+    for (int i = 0; i < count; ++i) p[off + i] = (float)(lim * (2.0 * unit01(seed, k++) - 1.0));
+  };
+  fill(0, h1 * nin, nin);
+  fill(h1 * nin + h1, h2 * h1, h1);
+  fill(h1 * nin + h1 + h2 * h1 + h2, h2, h2);
+  for (int i = 0; i < nin; ++i) p[h->mlp_np + nin + i] = 1.0f;  // identity normalisation
+  p[h->mlp_np + 2 * nin] = 2.0f;
+  p[h->mlp_np + 2 * nin + 1] = 1.0f;
+  HIP_TRY(h, hipMemcpy(h->mlp, p.data(), sizeof(float) * h->mlp_nb, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemset(h->mlp_m, 0, sizeof(float) * h->mlp_np));
+  HIP_TRY(h, hipMemset(h->mlp_v, 0, sizeof(float) * h->mlp_np));
+  h->adam_t = 0;
+  h->mlp_has_params = true;
+  return LBM_OK;
+}
+
+int lbm_mlp_set_params(lbm_handle* h, const float* params) {
+  if (!params) return fail(h, LBM_E_INVALID, "null params");
+  if (int rc = ensure_mlp(h)) return rc;
+  HIP_TRY(h, hipMemcpy(h->mlp, params, sizeof(float) * h->mlp_np, hipMemcpyHostToDevice));
+  h->mlp_has_params = true;
+  return LBM_OK;
+}
+
+int lbm_mlp_get_params(lbm_handle* h, float* params) {
+  if (!params) return fail(h, LBM_E_INVALID, "null params");
+  if (int rc = ensure_mlp(h)) return rc;
+  HIP_TRY(h, hipMemcpy(params, h->mlp, sizeof(float) * h->mlp_np, hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_mlp_set_normalization(lbm_handle* h, const float* in_mean, const float* in_scale,
+                              double out_mean, double out_scale) {
+  if (!in_mean || !in_scale) return fail(h, LBM_E_INVALID, "null normalisation");
+  if (int rc = ensure_mlp(h)) return rc;
+  std::vector<float> t(2 * h->mlp_nin + 2);
+  for (int i = 0; i < h->mlp_nin; ++i) {
+    t[i] = in_mean[i];
+    t[h->mlp_nin + i] = in_scale[i];
+  }
+  t[2 * h->mlp_nin] = (float)out_mean;
+  t[2 * h->mlp_nin + 1] = (float)out_scale;
+  HIP_TRY(h, hipMemcpy(h->mlp + h->mlp_np, t.data(), sizeof(float) * t.size(),
+                       hipMemcpyHostToDevice));
+  h->mlp_ready = true;
+  return LBM_OK;
+}
+
+int lbm_mlp_get_normalization(lbm_handle* h, float* in_mean, float* in_scale,
+                              double* out_mean, double* out_scale) {
+  if (int rc = ensure_mlp(h)) return rc;
+  std::vector<float> t(2 * h->mlp_nin + 2);
+  HIP_TRY(h, hipMemcpy(t.data(), h->mlp + h->mlp_np, sizeof(float) * t.size(),
+                       hipMemcpyDeviceToHost));
+  for (int i = 0; i < h->mlp_nin; ++i) {
+    if (in_mean) in_mean[i] = t[i];
+    if (in_scale) in_scale[i] = t[h->mlp_nin + i];
+  }
+  if (out_mean) *out_mean = t[2 * h->mlp_nin];
+  if (out_scale) *out_scale = t[2 * h->mlp_nin + 1];
+  return LBM_OK;
+}
+
+namespace {
+
+// Device buffers for one gradient / training pass over n samples.
+struct TrainBufs {
+  float *X = nullptr, *T = nullptr, *G = nullptr, *g = nullptr, *loss = nullptr;
+  int* idx = nullptr;
+  ~TrainBufs() {
+    for (void* p : {(void*)X, (void*)T, (void*)G, (void*)g, (void*)loss, (void*)idx})
+      if (p) (void)hipFree(p);
+  }
+};
+
+int targets_of(lbm_handle* h, const double* alpha, size_t n, std::vector<float>& t) {
+  std::vector<float> norm(2 * h->mlp_nin + 2);
+  HIP_TRY(h, hipMemcpy(norm.data(), h->mlp + h->mlp_np, sizeof(float) * norm.size(),
+                       hipMemcpyDeviceToHost));
+  const double om = norm[2 * h->mlp_nin], os = norm[2 * h->mlp_nin + 1];
+  t.resize(n);
+  for (size_t i = 0; i < n; ++i) t[i] = (float)((alpha[i] - om) / os);
+  return LBM_OK;
+}
+
+}  // namespace
+
+int lbm_mlp_gradient(lbm_handle* h, const float* x, const double* alpha, int n,
+                     float* grad, double* loss) {
+  if (!x || !alpha || !grad || n <= 0) return fail(h, LBM_E_INVALID, "bad samples");
+  if (int rc = ensure_mlp(h)) return rc;
+  std::vector<float> t;
+  if (int rc = targets_of(h, alpha, (size_t)n, t)) return rc;
+  TrainBufs b;
+  const int np = h->mlp_np;
+  HIP_TRY(h, hipMalloc(&b.X, sizeof(float) * (size_t)n * h->mlp_nin));
+  HIP_TRY(h, hipMalloc(&b.T, sizeof(float) * (size_t)n));
+  HIP_TRY(h, hipMalloc(&b.G, sizeof(float) * (size_t)n * np));
+  HIP_TRY(h, hipMalloc(&b.g, sizeof(float) * (size_t)np));
+  HIP_TRY(h, hipMalloc(&b.loss, sizeof(float) * (size_t)n));
+  HIP_TRY(h, hipMemcpy(b.X, x, sizeof(float) * (size_t)n * h->mlp_nin, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(b.T, t.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+  HIP_TRY(h, launch_mlp_sample_grad(h->cfg.lattice, h->mlp, b.X, b.T, nullptr, n, b.G,
+                                    b.loss, h->stream));
+  HIP_TRY(h, launch_mlp_sum(b.G, n, np, b.g, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipMemcpy(grad, b.g, sizeof(float) * np, hipMemcpyDeviceToHost));
+  if (loss) {
+    std::vector<float> l(n);
+    HIP_TRY(h, hipMemcpy(l.data(), b.loss, sizeof(float) * n, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (float v : l) s += v;
+    *loss = s / (2.0 * n);
+  }
+  return LBM_OK;
+}
+
+int lbm_mlp_predict(lbm_handle* h, const float* x, int n, double* alpha) {
+  if (!x || !alpha || n <= 0) return fail(h, LBM_E_INVALID, "bad samples");
+  if (int rc = ensure_mlp(h)) return rc;
+  float* dx = nullptr;
+  double* da = nullptr;
+  hipError_t e = hipMalloc(&dx, sizeof(float) * (size_t)n * h->mlp_nin);
+  if (e == hipSuccess) e = hipMalloc(&da, sizeof(double) * (size_t)n);
+  if (e == hipSuccess)
+    e = hipMemcpy(dx, x, sizeof(float) * (size_t)n * h->mlp_nin, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_mlp_forward(h->cfg.lattice, h->mlp, dx, n, da, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(alpha, da, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
+  if (dx) (void)hipFree(dx);
+  if (da) (void)hipFree(da);
+  if (e != hipSuccess) return hip_fail(h, e, "mlp predict");
+  return LBM_OK;
+}
+
+int lbm_sample_log_enable(lbm_handle* h, int64_t capacity, double fraction, uint32_t salt) {
+  if (int rc = need_mlp_lattice(h)) return rc;
+  if (capacity <= 0 || !(fraction > 0.0) || fraction > 1.0)
+    return fail(h, LBM_E_INVALID, "capacity > 0 and 0 < fraction <= 1 required");
+  if (h->log_x && capacity != h->log_cap) {
+    (void)hipFree(h->log_x);
+    (void)hipFree(h->log_alpha);
+    h->log_x = nullptr;
+    h->log_alpha = nullptr;
+  }
+  if (!h->log_x) {
+    HIP_TRY(h, hipMalloc(&h->log_x, sizeof(float) * (size_t)capacity * h->mlp_nin));
+    HIP_TRY(h, hipMalloc(&h->log_alpha, sizeof(double) * (size_t)capacity));
+  }
+  if (!h->log_count) HIP_TRY(h, hipMalloc(&h->log_count, sizeof(unsigned long long)));
+  HIP_TRY(h, hipMemsetAsync(h->log_count, 0, sizeof(unsigned long long), h->stream));
+  h->log_cap = capacity;
+  h->log_threshold = fraction >= 1.0 ? 0xFFFFFFFFu : (unsigned)(fraction * 4294967296.0);
+  h->log_salt = salt;
+  h->log_on = true;
+  return LBM_OK;
+}
+
+int lbm_sample_log_disable(lbm_handle* h) {
+  if (!h) return LBM_E_INVALID;
+  h->log_on = false;
+  return LBM_OK;
+}
+
+int lbm_sample_log_count(lbm_handle* h, int64_t* n) {
+  if (!h || !n) return LBM_E_INVALID;
+  *n = 0;
+  if (!h->log_count) return LBM_OK;
+  unsigned long long c = 0;
+  HIP_TRY(h, hipMemcpyAsync(&c, h->log_count, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  *n = (int64_t)std::min<unsigned long long>(c, (unsigned long long)h->log_cap);
+  return LBM_OK;
+}
+
+int lbm_sample_log_copy(lbm_handle* h, float* x, double* alpha, int64_t n) {
+  int64_t have = 0;
+  if (int rc = lbm_sample_log_count(h, &have)) return rc;
+  if (n > have) return fail(h, LBM_E_INVALID, "more samples requested than logged");
+  if (x) HIP_TRY(h, hipMemcpy(x, h->log_x, sizeof(float) * (size_t)n * h->mlp_nin,
+                              hipMemcpyDeviceToHost));
+  if (alpha) HIP_TRY(h, hipMemcpy(alpha, h->log_alpha, sizeof(double) * (size_t)n,
+                                  hipMemcpyDeviceToHost));
+  return LBM_OK;
+}
+
+int lbm_mlp_train(lbm_handle* h, int iterations, int batch, double lr, uint64_t seed,
+                  double* loss_first, double* loss_last) {
+  if (int rc = ensure_mlp(h)) return rc;
+  int64_t n = 0;
+  if (int rc = lbm_sample_log_count(h, &n)) return rc;
+  if (n < 2 || batch <= 0 || iterations <= 0)
+    return fail(h, LBM_E_STATE, "training needs logged samples (lbm_sample_log_enable + steps)");
+  // A network that was never initialised or loaded is all zeros, where ReLU
+  // units are dead: initialise it from the training seed.
+  if (!h->mlp_has_params)
+    if (int rc = lbm_mlp_init(h, seed)) return rc;
+  const int nin = h->mlp_nin, np = h->mlp_np;
+  // Normalisation from the samples: features standardised (scale floored at
+  // 1e-3 of the largest feature spread so conserved, ~zero moments stay
+  // quiet); alpha = 2 + std(alpha - 2) y.
+  std::vector<float> X((size_t)n * nin);
+  std::vector<double> al((size_t)n);
+  HIP_TRY(h, hipMemcpy(X.data(), h->log_x, sizeof(float) * X.size(), hipMemcpyDeviceToHost));
+  HIP_TRY(h, hipMemcpy(al.data(), h->log_alpha, sizeof(double) * al.size(), hipMemcpyDeviceToHost));
+  std::vector<double> mean(nin, 0.0), sd(nin, 0.0);
+  for (int64_t s = 0; s < n; ++s)
+    for (int k = 0; k < nin; ++k) mean[k] += X[(size_t)s * nin + k];
+  for (int k = 0; k < nin; ++k) mean[k] /= (double)n;
+  for (int64_t s = 0; s < n; ++s)
+    for (int k = 0; k < nin; ++k) {
+      const double d = X[(size_t)s * nin + k] - mean[k];
+      sd[k] += d * d;
+    }
+  double sdmax = 0.0;
+  for (int k = 0; k < nin; ++k) {
+    sd[k] = std::sqrt(sd[k] / (double)n);
+    sdmax = std::max(sdmax, sd[k]);
+  }
+  std::vector<float> fm(nin), fs(nin);
+  for (int k = 0; k < nin; ++k) {
+    fm[k] = (float)mean[k];
+    fs[k] = (float)(1.0 / std::max(sd[k], 1e-3 * sdmax + 1e-300));
+  }
+  double sa = 0.0;
+  for (double v : al) sa += (v - 2.0) * (v - 2.0);
+  const double out_scale = std::max(std::sqrt(sa / (double)n), 1e-12);
+  if (int rc = lbm_mlp_set_normalization(h, fm.data(), fs.data(), 2.0, out_scale)) return rc;
+  std::vector<float> t;
+  if (int rc = targets_of(h, al.data(), (size_t)n, t)) return rc;
+  const int B = (int)std::min<int64_t>(batch, n);
+  TrainBufs b;
+  HIP_TRY(h, hipMalloc(&b.T, sizeof(float) * (size_t)n));
+  HIP_TRY(h, hipMalloc(&b.G, sizeof(float) * (size_t)B * np));
+  HIP_TRY(h, hipMalloc(&b.g, sizeof(float) * (size_t)np));
+  HIP_TRY(h, hipMalloc(&b.loss, sizeof(float) * (size_t)B));
+  HIP_TRY(h, hipMalloc(&b.idx, sizeof(int) * (size_t)B));
+  HIP_TRY(h, hipMemcpy(b.T, t.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
+  // Seeded Fisher-Yates permutation, consumed B at a time.
+  std::vector<int> perm((size_t)n);
+  for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
+  uint64_t draw = 0;
+  auto shuffle = [&]() {
+    for (int64_t i = n - 1; i > 0; --i) {
+      const int64_t j = (int64_t)(mix64(seed, draw++) % (uint64_t)(i + 1));
+      std::swap(perm[i], perm[j]);
+    }
+  };
+  shuffle();
+  int64_t pos = 0;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  std::vector<float> lossv(B);
+  for (int it = 0; it < iterations; ++it) {
+    if (pos + B > n) {
+      shuffle();
+      pos = 0;
+    }
+    HIP_TRY(h, hipMemcpyAsync(b.idx, perm.data() + pos, sizeof(int) * B,
+                              hipMemcpyHostToDevice, h->stream));
+    pos += B;
+    HIP_TRY(h, launch_mlp_sample_grad(h->cfg.lattice, h->mlp, h->log_x, b.T, b.idx, B,
+                                      b.G, b.loss, h->stream));
+    HIP_TRY(h, launch_mlp_sum(b.G, B, np, b.g, h->stream));
+    h->adam_t += 1;
+    const float c1 = (float)(1.0 - std::pow(0.9, h->adam_t));
+    const float c2 = (float)(1.0 - std::pow(0.999, h->adam_t));
+    HIP_TRY(h, launch_mlp_adam(h->mlp, b.g, h->mlp_m, h->mlp_v, np, (float)lr, b1, b2,
+                               eps, c1, c2, h->stream));
+    if ((it == 0 && loss_first) || (it == iterations - 1 && loss_last)) {
+      HIP_TRY(h, hipMemcpyAsync(lossv.data(), b.loss, sizeof(float) * B,
+                                hipMemcpyDeviceToHost, h->stream));
+      HIP_TRY(h, hipStreamSynchronize(h->stream));
+      double s = 0.0;
+      for (float v : lossv) s += v;
+      if (it == 0 && loss_first) *loss_first = s / (2.0 * B);
+      if (it == iterations - 1 && loss_last) *loss_last = s / (2.0 * B);
+    }
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+int lbm_set_alpha_mode(lbm_handle* h, int mode, double residual_tol) {
+  if (!h) return LBM_E_INVALID;
+  if (h->cfg.collision != LBM_ELBM) return fail(h, LBM_E_STATE, "alpha mode belongs to ELBM");
+  if (mode != LBM_ALPHA_NEWTON && mode != LBM_ALPHA_LEARNED && mode != LBM_ALPHA_WARM_RESIDUAL)
+    return fail(h, LBM_E_INVALID, "alpha_mode must be NEWTON (0), LEARNED (1) or WARM_RESIDUAL (2)");
+  if (mode == LBM_ALPHA_LEARNED) {
+    if (int rc = need_mlp_lattice(h)) return rc;
+  }
+  if (mode != LBM_ALPHA_NEWTON && !(residual_tol > 0.0))
+    return fail(h, LBM_E_INVALID, "residual_tol must be positive");
+  h->alpha_mode = mode;
+  if (residual_tol > 0.0) h->residual_tol = residual_tol;
+  return LBM_OK;
+}
 
 }  // extern "C"

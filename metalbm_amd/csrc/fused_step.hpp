@@ -11,6 +11,7 @@
 // operation order (parity build); otherwise the shifted/FMA form is used.
 #pragma once
 
+#include "mlp.hpp"
 #include "step_kernels.hpp"
 
 namespace lbm {
@@ -116,18 +117,17 @@ __global__ void __launch_bounds__(BLOCK) k_bgk(const StepArgs a, int FSX) {
 
 // ---- entropic collision ------------------------------------------------------
 
-// ELBM alpha (Collision.h:351-375): the deviation test, alphaMax, then the
-// reference Newton-Raphson (EntropicStep.h:111-140) warm-started from the
-// cell's previous alpha.  f and n (= fneq) are the pulled populations and
-// their non-equilibrium parts in FP64.  G and G' share one log per term;
-// their values equal the reference's separate loops (EntropicStep.h:31-62).
-// REFDIV keeps the reference's divisions in the deviation tests.
+// ELBM alpha (Collision.h:351-375), first part: the deviation test
+// (Collision.h:284-303) and alphaMax (Collision.h:305-326).  Returns true when
+// the cell needs the root solve; otherwise alpha and branch are final
+// (0: deviation small -> 2; 1: alphaMax < 2 -> 0.95 alphaMax).  f and n
+// (= fneq) are the pulled populations and their non-equilibrium parts in
+// FP64.  REFDIV keeps the reference's divisions in the deviation test.
 template <class L, bool REFDIV>
-__device__ __forceinline__ double elbm_alpha(const double* f, const double* n,
-                                             double guess, int& branch,
-                                             int& iterations) {
+__device__ __forceinline__ bool elbm_prelude(const double* f, const double* n,
+                                             double& alpha, double& amax,
+                                             int& branch) {
   constexpr int Q = L::dimQ;
-  iterations = 0;
   bool small = true;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
@@ -140,9 +140,10 @@ __device__ __forceinline__ double elbm_alpha(const double* f, const double* n,
   });
   if (small) {
     branch = 0;
-    return 2.0;
+    alpha = 2.0;
+    return false;
   }
-  double amax = 2.5;
+  amax = 2.5;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     if (n[i] > 0) {
@@ -152,8 +153,23 @@ __device__ __forceinline__ double elbm_alpha(const double* f, const double* n,
   });
   if (amax < 2.) {
     branch = 1;
-    return 0.95 * amax;
+    alpha = 0.95 * amax;
+    return false;
   }
+  return true;
+}
+
+// The reference Newton-Raphson (EntropicStep.h:111-140 via solveAlpha,
+// Collision.h:328-349): warm start, |step| <= 1e-8, at most 50 iterations,
+// the iterate before the last step is returned, alpha = 2 when it fails or
+// leaves (1, alphaMax).  G and G' share one log per term; their values equal
+// the reference's separate loops (EntropicStep.h:31-62).
+template <class L>
+__device__ __forceinline__ double newton_reference(const double* f,
+                                                   const double* n,
+                                                   double guess, double amax,
+                                                   int& branch, int& iterations) {
+  constexpr int Q = L::dimQ;
   double A[Q];
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
@@ -161,6 +177,7 @@ __device__ __forceinline__ double elbm_alpha(const double* f, const double* n,
   });
   double x = guess, step = 0.0;
   bool ok = false;
+  iterations = 0;
   for (int it = 1; it <= 50; ++it) {
     x = x - step;
     iterations = it;
@@ -186,14 +203,33 @@ __device__ __forceinline__ double elbm_alpha(const double* f, const double* n,
   return x;
 }
 
+__device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
+  unsigned h = cell * 0x9E3779B1u ^ salt;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  return h ^ (h >> 16);
+}
+
 template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
   constexpr int Q = L::dimQ;
   constexpr bool REF = std::is_same_v<C, double> && !SH;
+  using Shape = MlpShape<L>;
   __shared__ unsigned int s_cnt[6];
+  extern __shared__ float s_mlp[];
   if (a.stats && threadIdx.x < 6) s_cnt[threadIdx.x] = 0u;
+  if constexpr (Shape::NIN > 0) {
+    if (a.alpha_mode == 1) {  // stage the network once per workgroup
+      using Dm = MlpDims<Shape::NIN, Shape::H1, Shape::H2>;
+      for (int t = threadIdx.x; t < Dm::NBLOCK; t += blockDim.x) s_mlp[t] = a.mlp[t];
+      __syncthreads();
+    }
+  }
   const CellPos c = cell_of_thread<L, BLOCK>(a);
   int branch = -1, its = 0;
+  bool first = false;
   if (c.active) {
     const int cell = self_index<L>(a, c);
     S* __restrict__ dst = static_cast<S*>(a.dst);
@@ -236,9 +272,40 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
       fd[i] = SH ? double(f[i]) + L::W[i] : double(f[i]);
       nd[i] = double(fneq[i]);
     });
-    const double guess = double(astate[cell]);
-    const double alpha = elbm_alpha<L, REF>(fd, nd, guess, branch, its);
-    astate[cell] = C(alpha);  // Algorithm.h:106
+    // The learned mode needs no warm start: its alpha state is neither read
+    // nor written (Algorithm.h:103-106 does both for the reference).
+    const bool warm = a.alpha_mode != 1;
+    double alpha, amax = 2.5;
+    if (elbm_prelude<L, REF>(fd, nd, alpha, amax, branch)) {
+      const double guess = warm ? double(astate[cell]) : 2.0;
+      if (a.alpha_mode == 0) {
+        alpha = newton_reference<L>(fd, nd, guess, amax, branch, its);
+      } else {
+        double start = guess;
+        if constexpr (Shape::NIN > 0) {
+          if (a.alpha_mode == 1) {
+            float m[Shape::NIN];
+            nonequilibrium_moments<L>(fneq, C(1) / rho, m);
+            start = mlp_alpha<Shape::NIN, Shape::H1, Shape::H2>(s_mlp, m);
+          }
+        }
+        alpha = residual_newton<L>(fd, nd, amax, start, a.residual_tol, branch,
+                                   its, first);
+      }
+      if constexpr (Shape::NIN > 0) {
+        if (a.log_cap > 0 && branch == 2 &&
+            sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {
+          const unsigned long long k = atomicAdd(a.log_count, 1ull);
+          if ((long long)k < a.log_cap) {
+            float m[Shape::NIN];
+            nonequilibrium_moments<L>(fneq, C(1) / rho, m);
+            for (int q = 0; q < Shape::NIN; ++q) a.log_x[k * Shape::NIN + q] = m[q];
+            a.log_alpha[k] = alpha;
+          }
+        }
+      }
+    }
+    if (warm) astate[cell] = C(alpha);  // Algorithm.h:106
     // tau = 1 / (alpha beta); collide with 1 / tau (Collision.h:240, 254).
     const double tau = 1.0 / (alpha * a.beta);
     const C rt = C(1.0 / tau);
@@ -284,6 +351,7 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
     if (branch >= 0) {
       atomicAdd(&s_cnt[branch], 1u);
       if (its) atomicAdd(&s_cnt[4], (unsigned)its);
+      if (first) atomicAdd(&s_cnt[5], 1u);
     }
     __syncthreads();
     if (threadIdx.x < 6 && s_cnt[threadIdx.x])

@@ -16,9 +16,16 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
   if constexpr (CO == (int)CollisionType::BGK)
     hipLaunchKernelGGL((k_bgk<L, S, C, FS, EQ, SH, 256>), dim3(grid),
                        dim3(block), 0, st, a, fsx);
-  else
+  else {
+    // Learned alpha stages the network in LDS (dynamic shared memory).
+    size_t lds = 0;
+    if constexpr (MlpShape<L>::NIN > 0) {
+      using Dm = MlpDims<MlpShape<L>::NIN, MlpShape<L>::H1, MlpShape<L>::H2>;
+      if (a.alpha_mode == 1) lds = sizeof(float) * Dm::NBLOCK;
+    }
     hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256>), dim3(grid),
-                       dim3(block), 0, st, a, fsx);
+                       dim3(block), lds, st, a, fsx);
+  }
   return hipGetLastError();
 }
 
@@ -52,13 +59,6 @@ void register_combo(Registry& r, int storage, int compute) {
   // There is no Exact equilibrium for D3Q19 (Equilibrium.h:36-126).
   if constexpr (L::Type != LatticeType::D3Q19)
     add_eq(std::integral_constant<int, (int)EquilibriumType::Exact>{});
-}
-
-template <class S, class C, bool SH, int... Vs>
-void register_all_lattices(Registry& r, int storage, int compute) {
-  register_combo<Lattice<LatticeType::D2Q9>, S, C, SH, Vs...>(r, storage, compute);
-  register_combo<Lattice<LatticeType::D3Q19>, S, C, SH, Vs...>(r, storage, compute);
-  register_combo<Lattice<LatticeType::D3Q27>, S, C, SH, Vs...>(r, storage, compute);
 }
 
 }  // namespace lbm

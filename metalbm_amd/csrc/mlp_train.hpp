@@ -1,0 +1,28 @@
+// mlp_train.hpp -- launchers of the alpha-predictor training kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lbm {
+
+// Network shape for a lattice (D2Q9: 9-32-32-1, D3Q27: 27-64-64-1);
+// nblock = parameters + normalisation floats.  Returns -1 if unsupported.
+int mlp_shape(int lattice, int* nin, int* h1, int* h2, int* nparam, int* nblock);
+
+// Per-sample gradients G[p][b] (parameter-major) and squared errors of the
+// samples idx[0..B) (idx may be NULL: samples 0..B); T holds the normalised
+// targets (alpha - out_mean) / out_scale.
+hipError_t launch_mlp_sample_grad(int lattice, const float* P, const float* X,
+                                  const float* T, const int* idx, int B,
+                                  float* G, float* loss, hipStream_t st);
+// g[p] = sum_b G[p][b] in order b = 0..B-1.
+hipError_t launch_mlp_sum(const float* G, int B, int np, float* g, hipStream_t st);
+// One Adam step; c1 = 1 - b1^t, c2 = 1 - b2^t.
+hipError_t launch_mlp_adam(float* P, const float* g, float* m, float* v, int np,
+                           float lr, float b1, float b2, float eps, float c1,
+                           float c2, hipStream_t st);
+// alpha_hat for raw feature rows X[n][nin].
+hipError_t launch_mlp_forward(int lattice, const float* P, const float* X, int n,
+                              double* alpha, hipStream_t st);
+
+}  // namespace lbm

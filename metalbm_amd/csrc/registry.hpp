@@ -38,8 +38,12 @@ struct Registry {
   }
 };
 
-void register_step_f64(Registry& r);
-void register_step_f32(Registry& r);
+void register_step_f64_d2q9(Registry& r);
+void register_step_f64_d3q19(Registry& r);
+void register_step_f64_d3q27(Registry& r);
+void register_step_f32_d2q9(Registry& r);
+void register_step_f32_d3q19(Registry& r);
+void register_step_f32_d3q27(Registry& r);
 
 const Registry& registry();
 

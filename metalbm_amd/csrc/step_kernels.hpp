@@ -48,6 +48,18 @@ struct StepArgs {
   double* F_out;        // [D][N]
   void* alpha_state;    // ELBM warm-start field (compute type)
   unsigned long long* stats;  // 6 ELBM counters or NULL
+  // Entropic alpha solver: 0 reference Newton, 1 learned start, 2 warm start,
+  // 1 and 2 with the residual acceptance (|G| <= residual_tol or certificate).
+  int alpha_mode;
+  double residual_tol;
+  const float* mlp;     // MLP parameter block (mode 1)
+  // (features, alpha) sample log of Newton-solved cells (log_cap 0: off)
+  float* log_x;
+  double* log_alpha;
+  unsigned long long* log_count;
+  long long log_cap;
+  unsigned log_threshold;  // sample when hash(cell, salt) < threshold
+  unsigned log_salt;
 };
 
 // Storage / compute codes (values of LBM_F64 / LBM_F32 / LBM_F16).

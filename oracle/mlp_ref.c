@@ -5,7 +5,8 @@
  * links it.  No reference counterpart exists (the reference solves alpha by
  * Newton only, include/metaLBM/Collision.h:328-349).
  *
- * Network: y = W3 tanh(W2 tanh(W1 x + b1) + b2) + b3, scalar output.
+ * Network: y = W3 relu(W2 relu(W1 x + b1) + b2) + b3, scalar output,
+ * relu(s) = max(s, 0) (derivative taken as 0 at s = 0).
  * Parameter vector layout (shared with the HIP kernels):
  *   W1[h1][nin], b1[h1], W2[h2][h1], b2[h2], W3[h2], b3[1].
  * Loss: L = 1/(2B) sum_b (y_b - t_b)^2.
@@ -29,12 +30,12 @@ static double forward1(const double *p, int nin, int h1, int h2,
   for (int j = 0; j < h1; ++j) {
     double s = b1[j];
     for (int k = 0; k < nin; ++k) s += W1[j * nin + k] * x[k];
-    a1[j] = tanh(s);
+    a1[j] = s > 0.0 ? s : 0.0;
   }
   for (int j = 0; j < h2; ++j) {
     double s = b2[j];
     for (int k = 0; k < h1; ++k) s += W2[j * h1 + k] * a1[k];
-    a2[j] = tanh(s);
+    a2[j] = s > 0.0 ? s : 0.0;
   }
   double y = b3[0];
   for (int k = 0; k < h2; ++k) y += W3[k] * a2[k];
@@ -82,7 +83,7 @@ double mlp_ref_grad(const double *p, int nin, int h1, int h2, int batch,
     gb3[0] += dy;
     for (int k = 0; k < h2; ++k) {
       gW3[k] += dy * a2[k];
-      d2[k] = dy * W3[k] * (1.0 - a2[k] * a2[k]);
+      d2[k] = a2[k] > 0.0 ? dy * W3[k] : 0.0;
       gb2[k] += d2[k];
     }
     for (int j = 0; j < h1; ++j) d1[j] = 0.0;
@@ -92,7 +93,7 @@ double mlp_ref_grad(const double *p, int nin, int h1, int h2, int batch,
         d1[j] += W2[k * h1 + j] * d2[k];
       }
     for (int j = 0; j < h1; ++j) {
-      d1[j] *= (1.0 - a1[j] * a1[j]);
+      d1[j] = a1[j] > 0.0 ? d1[j] : 0.0;
       gb1[j] += d1[j];
       for (int k = 0; k < nin; ++k) gW1[j * nin + k] += d1[j] * xb[k];
     }

@@ -1,0 +1,144 @@
+"""Learned entropic alpha on the GPU, checked against the FP64 oracle.
+
+* The training kernels' gradient and the inference code's prediction match
+  the FP64 reference network (oracle/mlp_ref.c) on the same weights.
+* End to end: (features, alpha) pairs are logged from the GPU's own Newton
+  solves, the network is trained on the GPU, and learned-alpha steps are
+  checked cell by cell: every Newton-branch cell satisfies the H-residual
+  bound |G(alpha)| <= tol with G evaluated by the oracle
+  (EntropicStep.h:31-45 restated in cpu_ref.c) on the same pulled state.
+* Ablation: the same residual Newton from the stored alpha (the method
+  without the network) on the same state; the test reports what the network
+  changes in evaluations per solved cell.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from metalbm_amd import (AlphaMode, Algorithm, CollisionType, Config, ForceType,
+                         ForcingSchemeType, LatticeType, Precision)
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("lat", [LatticeType.D2Q9, LatticeType.D3Q27])
+def test_mlp_gradient_and_forward_match_oracle(oracle, lat):
+    shape = (8, 8, 1) if lat == LatticeType.D2Q9 else (4, 4, 4)
+    cfg = Config(lattice=lat, collision=CollisionType.ELBM, nx=shape[0], ny=shape[1],
+                 nz=shape[2], tau=0.6)
+    say(f"case mlp grad {lat.name}")
+    with Algorithm(cfg) as a:
+        a.mlp_init(7)
+        nin, h1, h2, npar = a.mlp_shape()
+        rng = np.random.default_rng(3)
+        mean = (0.1 * rng.standard_normal(nin)).astype(np.float32)
+        scale = rng.uniform(0.5, 2.0, nin).astype(np.float32)
+        a.set_normalization(mean, scale, 2.0, 1e-3)
+        p = a.mlp_params()
+        # perturb biases so both ReLU sides are exercised
+        p = (p + 0.05 * rng.standard_normal(npar)).astype(np.float32)
+        a.set_mlp_params(p)
+        x = rng.standard_normal((96, nin)).astype(np.float32)
+        alpha = 2.0 + 1e-3 * rng.standard_normal(96)
+        g, loss = a.mlp_gradient(x, alpha)
+        pred = a.mlp_predict(x)
+    z = (x.astype(np.float64) - mean.astype(np.float64)) * scale.astype(np.float64)
+    t = (alpha - 2.0) / np.float64(np.float32(1e-3))
+    lref, gref = oracle.mlp_grad(p.astype(np.float64), nin, h1, h2, z, t)
+    assert abs(loss - lref) <= 1e-4 * abs(lref)
+    err = np.abs(g.astype(np.float64) - gref)
+    assert np.all(err <= 1e-4 * np.abs(gref).max() + 1e-4 * np.abs(gref)), err.max()
+    yref = oracle.mlp_forward(p.astype(np.float64), nin, h1, h2, z)
+    np.testing.assert_allclose(pred, 2.0 + np.float64(np.float32(1e-3)) * yref,
+                               rtol=0, atol=1e-3 * 1e-5 * max(1.0, np.abs(yref).max()))
+
+
+def pulled_state(o, oc, f):
+    """Pulled populations and fneq = f - feq(rho, u) per cell (oracle order)."""
+    c = o.velocities(oc.lattice)
+    Q, N = f.shape
+    shp = (oc.nx, oc.ny, oc.nz)
+    pad = (0,) if oc.d == 2 else ()
+    fp = np.stack([np.roll(f[i].reshape(shp), tuple(int(v) for v in c[i]) + pad,
+                           axis=(0, 1, 2)).ravel() for i in range(Q)])
+    rho = fp.sum(axis=0)
+    u = (c.T.astype(float) @ fp) / rho
+    feq = np.stack([o.equilibrium(oc, rho[k], u[:, k]) for k in range(N)], axis=1)
+    return fp, fp - feq
+
+
+CASES = [(LatticeType.D2Q9, (64, 64, 1), 1e-3, 16.0),
+         (LatticeType.D3Q27, (12, 12, 12), 3e-3, 6.0)]
+
+
+@pytest.mark.parametrize("lat,shape,amp,lam", CASES, ids=["D2Q9-64x64", "D3Q27-12^3"])
+def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
+    nx, ny, nz = shape
+    tol = 1e-10
+    cfg = Config(lattice=lat, collision=CollisionType.ELBM, forcing_scheme=ForcingSchemeType.Guo,
+                 force=ForceType.Kolmogorov, nx=nx, ny=ny, nz=nz, tau=0.50003,
+                 amplitude=(1e-5, 0, 0), wavelength=(lam, 1, 1), residual_tol=tol)
+    oc = oracle.Config(lattice=int(lat), collision=2, forcing_scheme=2, force=6, equilibrium=3,
+                       nx=nx, ny=ny, nz=nz, tau=0.50003, amplitude=(1e-5, 0, 0),
+                       wavelength=(lam, 1, 1))
+    say(f"case learned e2e {lat.name} {shape}")
+    with Algorithm(cfg) as a:
+        a.init_noise(1.0, amp, 42)
+        a.sample_log_enable(200000, 1.0, 5)
+        a.iterate(60)
+        a.sample_log_disable()
+        x, al = a.sample_log()
+        say(f"logged {len(al)} samples, alpha-2 std {np.std(al - 2):.3e}")
+        assert len(al) >= 1000
+        l0, l1 = a.mlp_train(1500, 1024, 2e-3, seed=1)
+        say(f"training loss {l0:.4e} -> {l1:.4e}")
+        assert l1 < 0.25 * l0
+        pred = a.mlp_predict(x[:20000])
+        err = np.abs(pred - al[:20000])
+        spread = np.std(al - 2)
+        say(f"prediction |err| median {np.median(err):.3e} p95 {np.percentile(err, 95):.3e} "
+            f"vs spread {spread:.3e}")
+        assert np.median(err) < 0.25 * spread
+        f0 = a.populations()
+        alpha0 = a.alpha()
+        res = {}
+        for mode in (AlphaMode.NEWTON, AlphaMode.WARM_RESIDUAL, AlphaMode.LEARNED):
+            a.set_populations(f0)
+            a.set_alpha(alpha0)
+            a.reset_elbm_stats()
+            a.set_alpha_mode(mode, tol)
+            a.iterate(1, store=True)
+            res[mode] = (a.store_fields()[2], a.elbm_stats(), a.populations())
+    fp, fneq = pulled_state(oracle, oc, f0)
+    newton_cells, worst = 0, 0.0
+    for k in range(fp.shape[1]):
+        _, b, _ = oracle.elbm_alpha(int(lat), fp[:, k], fneq[:, k], alpha0[k])
+        if b != oracle.ALPHA_NEWTON:
+            continue
+        newton_cells += 1
+        for mode in (AlphaMode.WARM_RESIDUAL, AlphaMode.LEARNED):
+            g = oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], res[mode][0][k])
+            worst = max(worst, abs(g))
+    say(f"{newton_cells} Newton cells, worst |G(alpha)| {worst:.3e} (tol {tol:.0e})")
+    assert newton_cells > 0.5 * fp.shape[1]
+    assert worst <= 1.001 * tol
+    for mode, (_, st, _) in res.items():
+        per = st["newton_iterations"] / max(1, st["newton"])
+        say(f"mode {mode.name}: {st}; evaluations per solved cell {per:.3f}")
+    st_l, st_w = res[AlphaMode.LEARNED][1], res[AlphaMode.WARM_RESIDUAL][1]
+    assert st_l["fallback"] == 0
+    # all three solvers find the same root to within the residual tolerance
+    fN, fL = res[AlphaMode.NEWTON][2], res[AlphaMode.LEARNED][2]
+    assert np.max(np.abs(fL - fN)) <= 1e-6 * np.max(np.abs(fN))
+    # what the network changes: evaluations per solved cell vs the same solver
+    # started from the stored alpha
+    ev_l = st_l["newton_iterations"] / max(1, st_l["newton"])
+    ev_w = st_w["newton_iterations"] / max(1, st_w["newton"])
+    say(f"network gain: {ev_w:.3f} -> {ev_l:.3f} evaluations per solved cell")
+    assert ev_l <= ev_w + 1e-9

@@ -64,6 +64,27 @@ WORKLOADS = {
                  forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
                  wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True),
         init=("noise", 1e-3, 42), steps=200),
+    "d2q9-elbm-2048-warm": dict(
+        desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, alpha by residual Newton "
+             "(|G| <= 1e-10 or Taylor certificate) from the stored alpha: the learned "
+             "solver without its network",
+        cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
+                 forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
+                 wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True,
+                 alpha_mode="WARM_RESIDUAL", residual_tol=1e-10),
+        init=("noise", 1e-3, 42), steps=200),
+    "d2q9-elbm-2048-learned": dict(
+        desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, alpha from the MLP "
+             "(9 normalised non-equilibrium moments -> 32 -> 32 -> 1, FP32 weights) trained "
+             "on the GPU on (features, alpha) logged from the first 1000 Newton steps, then "
+             "residual Newton (|G| <= 1e-10 or Taylor certificate), Newton fallback",
+        cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
+                 forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
+                 wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True,
+                 alpha_mode="NEWTON", residual_tol=1e-10),
+        init=("noise", 1e-3, 42), steps=200,
+        learn=dict(log_steps=1000, fraction=1e-4, capacity=500000, iters=3000, batch=4096,
+                   lr=2e-3)),
     "d2q9-bgk-128-f64": dict(
         desc="D2Q9 SRT BGK 128^2 FP64, tau 0.6, Guo + Kolmogorov F_x=1e-5 sin(2 pi 4 y/128), "
              "u0 noise +-1e-3 (seed 42) -- cache-resident parity config",
@@ -158,7 +179,8 @@ def make_config(spec):
     kw = dict(spec)
     conv = {"lattice": m.LatticeType, "collision": m.CollisionType,
             "forcing_scheme": m.ForcingSchemeType, "force": m.ForceType,
-            "equilibrium": m.EquilibriumType, "storage": m.Precision, "compute": m.Precision}
+            "equilibrium": m.EquilibriumType, "storage": m.Precision, "compute": m.Precision,
+            "alpha_mode": m.AlphaMode}
     for k, e in conv.items():
         if k in kw and isinstance(kw[k], str):
             kw[k] = e[kw[k]]
@@ -175,10 +197,11 @@ def init(a, spec):
         raise ValueError(kind)
 
 
-def bytes_per_cell(cfg, wl_cfg) -> int:
+def bytes_per_cell(cfg, wl_cfg, learned: bool = False) -> int:
     q = cfg.q
     b = 2 * q * SIZES[wl_cfg["storage"]]
-    if wl_cfg.get("collision") == "ELBM":
+    # The learned solver keeps no alpha state; the Newton ones read and write it.
+    if wl_cfg.get("collision") == "ELBM" and not learned:
         b += 2 * (8 if wl_cfg["compute"] == "F64" else 4)   # alpha warm-start state r+w
     return b
 
@@ -221,6 +244,45 @@ def cpu_baseline(wl, max_seconds=15.0):
                       f"same {cfg.nx}x{cfg.ny}x{cfg.nz} workload after 1 warm-up step"}
 
 
+def accuracy_study(nsteps: int, every: int, n: int) -> dict:
+    """Kinetic-energy decay of the D3Q27 Taylor-Green vortex (U0 = 0.05, tau =
+    0.51, n^3) with FP32 compute and shifted FP16 / FP32 / FP64 storage against
+    the FP64 reference-order build (bitwise equal to the FP64 oracle on every
+    tested configuration).  E(t) = (1/N) sum 1/2 rho |u_h|^2 every `every` steps."""
+    import numpy as np
+
+    import metalbm_amd as m
+    base = dict(lattice=m.LatticeType.D3Q27, nx=n, ny=n, nz=n, tau=0.51,
+                forcing_scheme=m.ForcingSchemeType.None_, force=m.ForceType.None_)
+    runs = {"f64-reference": dict(storage=m.Precision.F64, compute=m.Precision.F64, shifted=False),
+            "f16-shifted": dict(storage=m.Precision.F16, compute=m.Precision.F32, shifted=True),
+            "f32-shifted": dict(storage=m.Precision.F32, compute=m.Precision.F32, shifted=True),
+            "f64-shifted": dict(storage=m.Precision.F64, compute=m.Precision.F32, shifted=True),
+            "f32-unshifted": dict(storage=m.Precision.F32, compute=m.Precision.F32, shifted=False)}
+    series = {}
+    for name, prec in runs.items():
+        t0 = time.perf_counter()
+        with m.Algorithm(m.Config(**base, **prec)) as a:
+            a.init_taylor_green(1.0, 0.05)
+            e = []
+            for _ in range(nsteps // every):
+                a.iterate(every, store=True)
+                e.append(a.reduce()[1])
+        series[name] = np.array(e)
+        log(f"accuracy {name}: E({nsteps}) = {e[-1]:.10e} ({time.perf_counter() - t0:.1f} s)")
+    ref = series["f64-reference"]
+    out = {"workload": f"D3Q27 BGK {n}^3 Taylor-Green U0=0.05 tau=0.51, {nsteps} steps, "
+                       f"E every {every}", "E_reference_first_last": [ref[0], ref[-1]]}
+    for name, e in series.items():
+        if name == "f64-reference":
+            continue
+        rel = np.abs(e - ref) / np.abs(ref)
+        decay_err = np.abs((e[-1] - e[0]) - (ref[-1] - ref[0])) / abs(ref[-1] - ref[0])
+        out[name] = {"max_rel_E_error": float(rel.max()), "final_rel_E_error": float(rel[-1]),
+                     "rel_error_of_E_decay": float(decay_err)}
+    return out
+
+
 def traffic_of(name):
     p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(p):
@@ -239,7 +301,15 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 1 GiB (counter calibration)")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="D3Q27 Taylor-Green kinetic-energy decay, FP16/FP32/FP64 shifted "
+                         "storage vs the FP64 build (prints its own JSON line)")
+    ap.add_argument("--accuracy-n", type=int, default=384)
     args = ap.parse_args(argv)
+    if args.accuracy:
+        res = accuracy_study(args.steps or 2000, 100, args.accuracy_n)
+        print(json.dumps({"metric": "kinetic-energy decay error vs FP64", **res}))
+        return 0
     if args.copy_calibration:
         gbs = Hip().copy_calibration(1 << 30, 5)
         print(json.dumps({"copy_bytes_per_launch": 1 << 30, "launches": 5, "GB/s": round(gbs, 1)}))
@@ -260,8 +330,29 @@ def main(argv=None) -> int:
     s = hip.stream()
     a.set_stream(s.value)
     init(a, wl["init"])
+    learn = wl.get("learn")
+    learned = {}
+    if learn:
+        # Training data: (features, alpha) of the GPU's own Newton solves over
+        # the first log_steps steps, then Adam on the GPU.
+        t0 = time.perf_counter()
+        a.sample_log_enable(learn["capacity"], learn["fraction"], 1)
+        a.iterate(learn["log_steps"])
+        a.sample_log_disable()
+        a.synchronize()
+        t1 = time.perf_counter()
+        l0, l1 = a.mlp_train(learn["iters"], learn["batch"], learn["lr"], seed=1)
+        t2 = time.perf_counter()
+        n_samples = len(a.sample_log()[1])
+        a.set_alpha_mode(m.AlphaMode.LEARNED, wl["cfg"].get("residual_tol", 1e-10))
+        learned = {"logged_samples": n_samples, "log_steps": learn["log_steps"],
+                   "log_seconds": round(t1 - t0, 3), "train_iterations": learn["iters"],
+                   "train_seconds": round(t2 - t1, 3), "loss_first": l0, "loss_last": l1}
+        log(f"learned alpha: {learned}")
     a.iterate(W)
     a.synchronize()
+    if wl["cfg"].get("collision") == "ELBM":
+        a.reset_elbm_stats()
     e0, e1 = hip.event(), hip.event()
     hip.sync()
     t0 = time.perf_counter()
@@ -275,7 +366,7 @@ def main(argv=None) -> int:
     N = cfg.cells
     mlups = N * K / wall / 1e6
     kern_s = ev_ms / 1e3 / K
-    bpc = bytes_per_cell(cfg, wl["cfg"])
+    bpc = bytes_per_cell(cfg, wl["cfg"], learned=bool(learn))
     achieved = bpc * N / kern_s / 1e9
     log(f"wall {wall * 1e3:.2f} ms for {K} steps, events {ev_ms:.2f} ms, "
         f"{mlups:.1f} MLUPS, {achieved:.1f} GB/s algorithmic")
@@ -298,6 +389,10 @@ def main(argv=None) -> int:
     }
     if stats:
         out["elbm_stats"] = stats
+        out["elbm_stats"]["evaluations_per_solved_cell"] = round(
+            stats["newton_iterations"] / max(1, stats["newton"]), 4)
+    if learned:
+        out["learned_alpha"] = learned
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(wl)

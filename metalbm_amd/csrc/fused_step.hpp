@@ -212,35 +212,44 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
   return h ^ (h >> 16);
 }
 
-template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK>
+// Entropic step, one instantiation per alpha solver MODE (0: reference
+// Newton, 1: learned start + residual Newton, 2: residual Newton from the
+// stored alpha) so that no kernel carries the registers of another's solver.
+// Three phases so that the learned mode's wavefront-wide matrix-core
+// inference runs with every lane present: (1) pull, moments, fneq and the
+// alpha prelude per cell; (2) MODE 1 only: the network predicts alpha for
+// every lane's cell at once (its barriers are reached by all threads); (3) the
+// alpha solve, the collision and the stores per cell.
+template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
   constexpr int Q = L::dimQ;
   constexpr bool REF = std::is_same_v<C, double> && !SH;
   using Shape = MlpShape<L>;
   __shared__ unsigned int s_cnt[6];
-  extern __shared__ float s_mlp[];
+  extern __shared__ __align__(16) char s_lds[];
   if (a.stats && threadIdx.x < 6) s_cnt[threadIdx.x] = 0u;
-  if constexpr (Shape::NIN > 0) {
-    if (a.alpha_mode == 1) {  // stage the network once per workgroup
-      using Dm = MlpDims<Shape::NIN, Shape::H1, Shape::H2>;
-      for (int t = threadIdx.x; t < Dm::NBLOCK; t += blockDim.x) s_mlp[t] = a.mlp[t];
+  static_assert(MODE != 1 || Shape::NIN > 0, "no alpha network for this lattice");
+  if constexpr (MODE == 1) {
+    {  // stage the network once per workgroup
+      stage_mfma_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp, s_lds);
       __syncthreads();
     }
   }
   const CellPos c = cell_of_thread<L, BLOCK>(a);
   int branch = -1, its = 0;
-  bool first = false;
+  bool first = false, solve = false;
+  int cell = 0;
+  C f[Q], fneq[Q], eqv[Q], F[3];
+  C rho = C(1), u[3] = {C(0), C(0), C(0)}, u2 = C(0), drho = C(0);
+  double alpha = 2.0, amax = 2.5;
+  // The learned mode needs no warm start: its alpha state is neither read
+  // nor written (Algorithm.h:103-106 does both for the reference).
+  constexpr bool warm = MODE != 1;
+  // ---- phase 1 ----
   if (c.active) {
-    const int cell = self_index<L>(a, c);
-    S* __restrict__ dst = static_cast<S*>(a.dst);
-    C* __restrict__ astate = static_cast<C*>(a.alpha_state);
-    C f[Q];
+    cell = self_index<L>(a, c);
     pull<L, S, C>(a, c, f);
-    C F[3];
     force_of_cell<L, C>(a, c, F);
-    const C gc = C(a.guo_coef);
-    C rho, u[3] = {C(0), C(0), C(0)}, u2, drho = C(0);
-    C fneq[Q], eqv[Q];
     if constexpr (REF) {
       moments_ref<L>(f, rho, u, u2);
       const RefEquilibrium<L, EQ, C> E(rho, u, u2);
@@ -272,40 +281,52 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
       fd[i] = SH ? double(f[i]) + L::W[i] : double(f[i]);
       nd[i] = double(fneq[i]);
     });
-    // The learned mode needs no warm start: its alpha state is neither read
-    // nor written (Algorithm.h:103-106 does both for the reference).
-    const bool warm = a.alpha_mode != 1;
-    double alpha, amax = 2.5;
-    if (elbm_prelude<L, REF>(fd, nd, alpha, amax, branch)) {
-      const double guess = warm ? double(astate[cell]) : 2.0;
-      if (a.alpha_mode == 0) {
-        alpha = newton_reference<L>(fd, nd, guess, amax, branch, its);
-      } else {
-        double start = guess;
-        if constexpr (Shape::NIN > 0) {
-          if (a.alpha_mode == 1) {
-            float m[Shape::NIN];
-            nonequilibrium_moments<L>(fneq, C(1) / rho, m);
-            start = mlp_alpha<Shape::NIN, Shape::H1, Shape::H2>(s_mlp, m);
-          }
-        }
+    solve = elbm_prelude<L, REF>(fd, nd, alpha, amax, branch);
+  }
+  // ---- phase 2 ----
+  double start = 2.0;
+  float m[Shape::NIN > 0 ? Shape::NIN : 1];
+  if constexpr (Shape::NIN > 0) {
+    const bool need_m = solve && (MODE == 1 || a.log_cap > 0);
+    if (need_m) {
+      nonequilibrium_moments<L>(fneq, C(1) / rho, m);
+    } else {
+#pragma unroll
+      for (int k = 0; k < Shape::NIN; ++k) m[k] = 0.0f;
+    }
+    if constexpr (MODE == 1)
+      start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2>(s_lds, m);
+  }
+  // ---- phase 3 ----
+  if (c.active) {
+    S* __restrict__ dst = static_cast<S*>(a.dst);
+    C* __restrict__ astate = static_cast<C*>(a.alpha_state);
+    if (solve) {
+      double fd[Q], nd[Q];
+      static_for<Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        fd[i] = SH ? double(f[i]) + L::W[i] : double(f[i]);
+        nd[i] = double(fneq[i]);
+      });
+      if constexpr (warm) start = double(astate[cell]);
+      if constexpr (MODE == 0)
+        alpha = newton_reference<L>(fd, nd, start, amax, branch, its);
+      else
         alpha = residual_newton<L>(fd, nd, amax, start, a.residual_tol, branch,
                                    its, first);
-      }
       if constexpr (Shape::NIN > 0) {
         if (a.log_cap > 0 && branch == 2 &&
             sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {
           const unsigned long long k = atomicAdd(a.log_count, 1ull);
           if ((long long)k < a.log_cap) {
-            float m[Shape::NIN];
-            nonequilibrium_moments<L>(fneq, C(1) / rho, m);
             for (int q = 0; q < Shape::NIN; ++q) a.log_x[k * Shape::NIN + q] = m[q];
             a.log_alpha[k] = alpha;
           }
         }
       }
     }
-    if (warm) astate[cell] = C(alpha);  // Algorithm.h:106
+    if constexpr (warm) astate[cell] = C(alpha);  // Algorithm.h:106
+    const C gc = C(a.guo_coef);
     // tau = 1 / (alpha beta); collide with 1 / tau (Collision.h:240, 254).
     const double tau = 1.0 / (alpha * a.beta);
     const C rt = C(1.0 / tau);

@@ -16,15 +16,19 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
   if constexpr (CO == (int)CollisionType::BGK)
     hipLaunchKernelGGL((k_bgk<L, S, C, FS, EQ, SH, 256>), dim3(grid),
                        dim3(block), 0, st, a, fsx);
-  else {
-    // Learned alpha stages the network in LDS (dynamic shared memory).
-    size_t lds = 0;
-    if constexpr (MlpShape<L>::NIN > 0) {
-      using Dm = MlpDims<MlpShape<L>::NIN, MlpShape<L>::H1, MlpShape<L>::H2>;
-      if (a.alpha_mode == 1) lds = sizeof(float) * Dm::NBLOCK;
-    }
-    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256>), dim3(grid),
-                       dim3(block), lds, st, a, fsx);
+  else if (a.alpha_mode == 0)
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 0>), dim3(grid),
+                       dim3(block), 0, st, a, fsx);
+  else if (a.alpha_mode == 2)
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 2>), dim3(grid),
+                       dim3(block), 0, st, a, fsx);
+  else if constexpr (MlpShape<L>::NIN > 0) {
+    // The learned solver stages the network in LDS (dynamic shared memory).
+    using M = MfmaLayout<MlpShape<L>::NIN, MlpShape<L>::H1, MlpShape<L>::H2>;
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 1>), dim3(grid),
+                       dim3(block), M::BYTES, st, a, fsx);
+  } else {
+    return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -73,31 +73,125 @@ __device__ __forceinline__ void nonequilibrium_moments(const T* fneq, T rrho,
   });
 }
 
-// One forward pass for one cell, parameters in LDS (uniform reads).
+// ---- wavefront inference on the matrix cores --------------------------------
+//
+// The fused kernel runs the network for all 64 cells of a wavefront with
+// v_mfma_f32_16x16x16f16 (f16 operands, FP32 accumulation).  Cells are the N
+// dimension, so a layer's accumulator (lane l: cell l%16 of the tile, rows
+// 4(l/16)+j) is already the next layer's B operand (lane l: column l%16,
+// k = 4(l/16)+j): H^T = relu(W . X^T + b) chains without a transpose.
+// LDS layout (bytes), staged once per workgroup from the FP32 block:
+//   W1h[H1][KP] W2h[H2][H1] W3h[16][H2] (f16, zero padded; W3h row 0 = W3)
+//   b1[H1] b2[H2] b3 mean[NIN] scale[NIN] out[2] (f32)
+//   X[4 waves][64][KP] (f16 standardised features)  Y[4 waves][64] (f32)
 template <int NIN, int H1, int H2>
-__device__ __forceinline__ double mlp_alpha(const float* __restrict__ P,
-                                            const float* m) {
+struct MfmaLayout {
+  static constexpr int KP = NIN <= 16 ? 16 : 32;
+  static constexpr int W1H = 0;
+  static constexpr int W2H = W1H + H1 * KP * 2;
+  static constexpr int W3H = W2H + H2 * H1 * 2;
+  static constexpr int F32 = W3H + 16 * H2 * 2;
+  static constexpr int B1 = F32, B2 = B1 + 4 * H1, B3 = B2 + 4 * H2, MEAN = B3 + 4,
+                       SCALE = MEAN + 4 * NIN, OUT = SCALE + 4 * NIN;
+  static constexpr int X = ((OUT + 8 + 15) / 16) * 16;
+  static constexpr int Y = X + 4 * 64 * KP * 2;
+  static constexpr int BYTES = Y + 4 * 64 * 4;
+};
+
+typedef _Float16 mfma_h4 __attribute__((ext_vector_type(4)));
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+// Cooperative staging of the FP32 parameter block into the LDS layout.
+template <int NIN, int H1, int H2>
+__device__ __forceinline__ void stage_mfma_network(const float* __restrict__ P,
+                                                   char* lds) {
   using Dm = MlpDims<NIN, H1, H2>;
-  float z[NIN];
-#pragma unroll
-  for (int k = 0; k < NIN; ++k) z[k] = (m[k] - P[Dm::MEAN + k]) * P[Dm::SCALE + k];
-  float a1[H1];
-#pragma unroll
-  for (int j = 0; j < H1; ++j) {
-    float s = P[Dm::B1 + j];
-#pragma unroll
-    for (int k = 0; k < NIN; ++k) s = fmaf(P[Dm::W1 + j * NIN + k], z[k], s);
-    a1[j] = fmaxf(s, 0.0f);
+  using M = MfmaLayout<NIN, H1, H2>;
+  _Float16* w1 = reinterpret_cast<_Float16*>(lds + M::W1H);
+  _Float16* w2 = reinterpret_cast<_Float16*>(lds + M::W2H);
+  _Float16* w3 = reinterpret_cast<_Float16*>(lds + M::W3H);
+  float* fl = reinterpret_cast<float*>(lds + M::F32);
+  for (int t = threadIdx.x; t < H1 * M::KP; t += blockDim.x) {
+    const int r = t / M::KP, k = t - r * M::KP;
+    w1[t] = k < NIN ? _Float16(P[Dm::W1 + r * NIN + k]) : _Float16(0.0f);
   }
-  float y = P[Dm::B3];
+  for (int t = threadIdx.x; t < H2 * H1; t += blockDim.x) w2[t] = _Float16(P[Dm::W2 + t]);
+  for (int t = threadIdx.x; t < 16 * H2; t += blockDim.x)
+    w3[t] = t < H2 ? _Float16(P[Dm::W3 + t]) : _Float16(0.0f);
+  for (int t = threadIdx.x; t < H1; t += blockDim.x) fl[t] = P[Dm::B1 + t];
+  for (int t = threadIdx.x; t < H2; t += blockDim.x) fl[H1 + t] = P[Dm::B2 + t];
+  for (int t = threadIdx.x; t < 2 * NIN + 3; t += blockDim.x)
+    fl[H1 + H2 + t] = P[Dm::B3 + t];  // b3, mean[NIN], scale[NIN], out[2]
+}
+
+// alpha_hat for every lane's cell (m: raw features; lanes without a cell pass
+// zeros).  Must be reached by all threads of the workgroup (two barriers).
+template <int NIN, int H1, int H2>
+__device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
+  using M = MfmaLayout<NIN, H1, H2>;
+  constexpr int KP = M::KP;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const float* fl = reinterpret_cast<const float*>(lds + M::F32);
+  const float* b1 = fl;
+  const float* b2 = fl + H1;
+  const float b3 = fl[H1 + H2];
+  const float* mean = fl + H1 + H2 + 1;
+  const float* scale = mean + NIN;
+  _Float16* X = reinterpret_cast<_Float16*>(lds + M::X) + wv * 64 * KP;
+  float* Y = reinterpret_cast<float*>(lds + M::Y) + wv * 64;
 #pragma unroll
-  for (int j = 0; j < H2; ++j) {
-    float s = P[Dm::B2 + j];
+  for (int k4 = 0; k4 < KP; k4 += 4) {
+    mfma_h4 z;
 #pragma unroll
-    for (int k = 0; k < H1; ++k) s = fmaf(P[Dm::W2 + j * H1 + k], a1[k], s);
-    y = fmaf(P[Dm::W3 + j], fmaxf(s, 0.0f), y);
+    for (int j = 0; j < 4; ++j) {
+      const int k = k4 + j;
+      z[j] = k < NIN ? _Float16((m[k] - mean[k]) * scale[k]) : _Float16(0.0f);
+    }
+    *reinterpret_cast<mfma_h4*>(X + lane * KP + k4) = z;
   }
-  return double(P[Dm::OUT]) + double(P[Dm::OUT + 1]) * double(y);
+  __syncthreads();
+  const _Float16* w1 = reinterpret_cast<const _Float16*>(lds + M::W1H);
+  const _Float16* w2 = reinterpret_cast<const _Float16*>(lds + M::W2H);
+  const _Float16* w3 = reinterpret_cast<const _Float16*>(lds + M::W3H);
+  const int col = lane & 15, kq = 4 * (lane >> 4);
+  auto frag = [](const _Float16* p) { return *reinterpret_cast<const mfma_h4*>(p); };
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {  // 16-cell tiles of the wavefront
+    mfma_h4 h1[H1 / 16];
+#pragma unroll
+    for (int rt = 0; rt < H1 / 16; ++rt) {
+      mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < KP / 16; ++kb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
+            frag(w1 + (rt * 16 + col) * KP + kb * 16 + kq),
+            frag(X + (t * 16 + col) * KP + kb * 16 + kq), acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        h1[rt][j] = _Float16(fmaxf(acc[j] + b1[rt * 16 + kq + j], 0.0f));
+    }
+    mfma_h4 h2[H2 / 16];
+#pragma unroll
+    for (int rt = 0; rt < H2 / 16; ++rt) {
+      mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < H1 / 16; ++kb)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
+            frag(w2 + (rt * 16 + col) * H1 + kb * 16 + kq), h1[kb], acc, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        h2[rt][j] = _Float16(fmaxf(acc[j] + b2[rt * 16 + kq + j], 0.0f));
+    }
+    mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < H2 / 16; ++kb)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
+          frag(w3 + col * H2 + kb * 16 + kq), h2[kb], acc, 0, 0, 0);
+    if (lane < 16) Y[t * 16 + lane] = acc[0];  // row 0 = the output unit
+  }
+  __syncthreads();
+  const float y = Y[lane] + b3;
+  return double(fl[H1 + H2 + 1 + 2 * NIN]) + double(fl[H1 + H2 + 2 + 2 * NIN]) * double(y);
 }
 
 // ---- residual Newton with a Taylor certificate ------------------------------
@@ -112,32 +206,72 @@ __device__ __forceinline__ double mlp_alpha(const float* __restrict__ P,
 // an evaluated point, or when the certificate of the next point is <= tol.
 // Outcome codes: 2 Newton accepted, 3 fallback (alpha = 2); `first` reports
 // acceptance from the starting point's single evaluation (plus certificate).
+
+// log1p(y) for |y| <= kSeriesMax by its Taylor series to degree 12
+// (truncation < |y|^13 / 13 < 1e-20 |y|): 12 FMAs instead of a library call.
+constexpr double kSeriesMax = 0.03;
+__device__ __forceinline__ double log1p_small(double y) {
+  double p = -1.0 / 12.0;
+  p = fma(p, y, 1.0 / 11.0);
+  p = fma(p, y, -1.0 / 10.0);
+  p = fma(p, y, 1.0 / 9.0);
+  p = fma(p, y, -1.0 / 8.0);
+  p = fma(p, y, 1.0 / 7.0);
+  p = fma(p, y, -1.0 / 6.0);
+  p = fma(p, y, 1.0 / 5.0);
+  p = fma(p, y, -1.0 / 4.0);
+  p = fma(p, y, 1.0 / 3.0);
+  p = fma(p, y, -1.0 / 2.0);
+  p = fma(p, y, 1.0);
+  return p * y;
+}
+// 1/d for d near 1 (|1 - d| <= kSeriesMax) to full precision: hardware
+// reciprocal estimate plus two Newton-Raphson corrections.
+__device__ __forceinline__ double recip_near_one(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// n / f for f > 0 in the normal range: reciprocal estimate, two
+// Newton-Raphson corrections and a final residual correction of the quotient
+// (error of a few ulp, well inside the solver's rounding allowance).
+__device__ __forceinline__ double div_refined(double n, double f) {
+  double r = __builtin_amdgcn_rcp(f);
+  r = fma(r, fma(-f, r, 1.0), r);
+  r = fma(r, fma(-f, r, 1.0), r);
+  const double q = n * r;
+  return fma(r, fma(-f, q, n), q);
+}
+
+// G, G' and -G'' at alpha = a (t_i = a x_i), with log1p and 1/(1 - t) by
+// series and refined reciprocal when every |t_i| <= kSeriesMax, else by the
+// library.  The choice is made once per evaluation, not per term, so the two
+// variants are never both computed.
 template <class L>
-__device__ __forceinline__ double residual_newton(const double* f,
-                                                  const double* n, double amax,
-                                                  double start, double tol,
-                                                  int& branch, int& evals,
-                                                  bool& first) {
+__device__ __forceinline__ void residual_terms(const double* f, const double* n,
+                                               const double* x, const double* A,
+                                               double a, double& g, double& dg,
+                                               double& h) {
   constexpr int Q = L::dimQ;
-  double x[Q], A[Q];
+  double tmax = 0.0;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    x[i] = n[i] / f[i];
-    A[i] = log(f[i] * (1.0 / L::W[i]));
+    tmax = fmax(tmax, fabs(a * x[i]));
   });
-  auto d2 = [&](double a) {
-    double s = 0.0;
+  g = 0.0;
+  dg = 0.0;
+  h = 0.0;
+  if (tmax <= kSeriesMax) {
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      s += n[i] * x[i] / (1.0 - a * x[i]);
+      const double t = a * x[i];
+      const double l1 = log1p_small(-t);
+      g += f[i] * (t * A[i] - (1.0 - t) * l1);
+      dg += n[i] * (1.0 + A[i] + l1);
+      h += n[i] * x[i] * recip_near_one(1.0 - t);
     });
-    return s;  // = -G''(a) >= 0
-  };
-  double a = (start > 1.0 && start < amax) ? start : 2.0;
-  evals = 0;
-  first = false;
-  for (int it = 0; it < 50; ++it) {
-    double g = 0.0, dg = 0.0, h = 0.0;
+  } else {
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const double t = a * x[i];
@@ -146,6 +280,72 @@ __device__ __forceinline__ double residual_newton(const double* f,
       dg += n[i] * (1.0 + A[i] + l1);
       h += n[i] * x[i] / (1.0 - t);
     });
+  }
+}
+
+// -G''(a) = sum n_i x_i / (1 - a x_i) >= 0, as in residual_terms.
+template <class L>
+__device__ __forceinline__ double residual_curvature(const double* n,
+                                                     const double* x, double a) {
+  constexpr int Q = L::dimQ;
+  double tmax = 0.0;
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    tmax = fmax(tmax, fabs(a * x[i]));
+  });
+  double s = 0.0;
+  if (tmax <= kSeriesMax) {
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      s += n[i] * x[i] * recip_near_one(1.0 - a * x[i]);
+    });
+  } else {
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      s += n[i] * x[i] / (1.0 - a * x[i]);
+    });
+  }
+  return s;
+}
+
+// Residual Newton for G(alpha) = 0 (G in the log1p form of EntropicStep.h:
+// 31-45): accepts alpha once |G| <= tol, or after a Newton step s whose
+// Taylor remainder bound 1/2 s^2 max|G''| (G'' is convex in alpha, so its
+// maximum over the step lies at an end point) is <= tol, with a relative
+// rounding allowance.  Returns 2 with branch 3 if an iterate leaves
+// (1, amax) or 50 evaluations do not converge.
+template <class L>
+__device__ __forceinline__ double residual_newton(const double* f,
+                                                  const double* n, double amax,
+                                                  double start, double tol,
+                                                  int& branch, int& evals,
+                                                  bool& first) {
+  constexpr int Q = L::dimQ;
+  double x[Q], A[Q];
+  bool small_a = true;
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    x[i] = div_refined(n[i], f[i]);
+    A[i] = fma(f[i], 1.0 / L::W[i], -1.0);  // f/w - 1
+    small_a = small_a && fabs(A[i]) <= kSeriesMax;
+  });
+  if (small_a) {
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      A[i] = log1p_small(A[i]);  // ln(f/w)
+    });
+  } else {
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      A[i] = log1p(A[i]);
+    });
+  }
+  double a = (start > 1.0 && start < amax) ? start : 2.0;
+  evals = 0;
+  first = false;
+  for (int it = 0; it < 50; ++it) {
+    double g, dg, h;
+    residual_terms<L>(f, n, x, A, a, g, dg, h);
     ++evals;
     if (fabs(g) <= tol) {
       branch = 2;
@@ -155,7 +355,8 @@ __device__ __forceinline__ double residual_newton(const double* f,
     const double s = g / dg;
     const double an = a - s;
     if (!(an > 1.0 && an < amax)) break;
-    const double bound = 0.5 * s * s * fmax(h, d2(an)) * (1.0 + 1e-6);
+    const double bound =
+        0.5 * s * s * fmax(h, residual_curvature<L>(n, x, an)) * (1.0 + 1e-6);
     if (bound <= tol) {
       branch = 2;
       first = it == 0;

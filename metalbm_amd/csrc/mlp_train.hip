@@ -85,15 +85,20 @@ k_mlp_adam(float* __restrict__ P, const float* __restrict__ g,
   P[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + eps);
 }
 
+// Batch inference with the step kernel's own matrix-core code (f16 operands,
+// FP32 accumulation), so predictions equal what the fused step starts from.
 template <int NIN, int H1, int H2>
 __global__ void __launch_bounds__(256)
 k_mlp_forward(const float* __restrict__ P, const float* __restrict__ X, int n,
               double* __restrict__ alpha) {
+  extern __shared__ __align__(16) char s_lds[];
+  stage_mfma_network<NIN, H1, H2>(P, s_lds);
+  __syncthreads();
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
   float m[NIN];
-  for (int k = 0; k < NIN; ++k) m[k] = X[(size_t)s * NIN + k];
-  alpha[s] = mlp_alpha<NIN, H1, H2>(P, m);
+  for (int k = 0; k < NIN; ++k) m[k] = s < n ? X[(size_t)s * NIN + k] : 0.0f;
+  const double y = mlp_alpha_wave<NIN, H1, H2>(s_lds, m);
+  if (s < n) alpha[s] = y;
 }
 
 template <class F>
@@ -156,9 +161,10 @@ hipError_t launch_mlp_adam(float* P, const float* g, float* m, float* v, int np,
 hipError_t launch_mlp_forward(int lattice, const float* P, const float* X, int n,
                               double* alpha, hipStream_t st) {
   return by_lattice(lattice, [&](auto nin, auto h1, auto h2) {
+    using M = MfmaLayout<decltype(nin)::value, decltype(h1)::value, decltype(h2)::value>;
     hipLaunchKernelGGL((k_mlp_forward<decltype(nin)::value, decltype(h1)::value,
                                       decltype(h2)::value>),
-                       dim3((n + 255) / 256), dim3(256), 0, st, P, X, n, alpha);
+                       dim3((n + 255) / 256), dim3(256), M::BYTES, st, P, X, n, alpha);
     return hipGetLastError();
   });
 }

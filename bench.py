@@ -85,6 +85,32 @@ WORKLOADS = {
         init=("noise", 1e-3, 42), steps=200,
         learn=dict(log_steps=1000, fraction=1e-4, capacity=500000, iters=3000, batch=4096,
                    lr=2e-3)),
+    "d3q27-elbm-256-newton": dict(
+        desc="D3Q27 SRT Entropic 256^3 decaying turbulence, nu=2e-5 (tau=0.50006), no force, "
+             "solenoidal random-phase u0 with E(k) ~ k^4 exp(-2 (k/4)^2) on shells 1..16 "
+             "(seed 42), per-component rms 0.1 cs, FP32 compute + FP64 reference Newton alpha",
+        cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
+                 forcing_scheme="None_", force="None_", storage="F32", compute="F32",
+                 shifted=True),
+        init=("spectral", 0.1 / 3 ** 0.5, 16, 4.0, 42), steps=50),
+    "d3q27-elbm-256-warm": dict(
+        desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, alpha by residual Newton "
+             "(|G| <= 1e-10 or Taylor certificate) from the stored alpha",
+        cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
+                 forcing_scheme="None_", force="None_", storage="F32", compute="F32",
+                 shifted=True, alpha_mode="WARM_RESIDUAL", residual_tol=1e-10),
+        init=("spectral", 0.1 / 3 ** 0.5, 16, 4.0, 42), steps=50),
+    "d3q27-elbm-256-learned": dict(
+        desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, alpha from the MLP "
+             "(27 normalised non-equilibrium moments -> 64 -> 64 -> 1, BF16 weights on the "
+             "matrix cores, FP32 accumulate) trained on the GPU on (features, alpha) logged "
+             "from the first 200 Newton steps, then residual Newton, Newton fallback",
+        cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
+                 forcing_scheme="None_", force="None_", storage="F32", compute="F32",
+                 shifted=True, alpha_mode="NEWTON", residual_tol=1e-10),
+        init=("spectral", 0.1 / 3 ** 0.5, 16, 4.0, 42), steps=50,
+        learn=dict(log_steps=200, fraction=1e-4, capacity=500000, iters=3000, batch=4096,
+                   lr=2e-3)),
     "d2q9-bgk-128-f64": dict(
         desc="D2Q9 SRT BGK 128^2 FP64, tau 0.6, Guo + Kolmogorov F_x=1e-5 sin(2 pi 4 y/128), "
              "u0 noise +-1e-3 (seed 42) -- cache-resident parity config",
@@ -193,6 +219,8 @@ def init(a, spec):
         a.init_noise(1.0, spec[1], spec[2])
     elif kind == "tgv":
         a.init_taylor_green(1.0, spec[1])
+    elif kind == "spectral":
+        a.init_spectral(1.0, spec[1], spec[2], spec[3], spec[4])
     else:
         raise ValueError(kind)
 
@@ -225,10 +253,13 @@ def cpu_baseline(wl, max_seconds=15.0):
                    wavelength=c.get("wavelength", (1, 1, 1)))
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     o.lib().ref_set_threads(threads)
-    if wl["init"][0] == "noise":
-        u = o.noise_velocity(cfg, wl["init"][2], wl["init"][1])
+    spec = wl["init"]
+    if spec[0] == "noise":
+        u = o.noise_velocity(cfg, spec[2], spec[1])
+    elif spec[0] == "spectral":
+        u = o.spectral_velocity(cfg, spec[1], spec[2], spec[3], spec[4])
     else:
-        u = o.taylor_green(cfg, wl["init"][1])
+        u = o.taylor_green(cfg, spec[1])
     f = o.init_equilibrium(cfg, np.ones(cfg.cells), u)
     del u
     t0 = time.perf_counter()

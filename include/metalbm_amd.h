@@ -145,6 +145,26 @@ int lbm_init_noise(lbm_handle *h, double rho0, double amp, uint64_t seed);
  * -cos x sin y cos z, 0) (2D: without the z factor), x = 2 pi i / n. */
 int lbm_init_taylor_green(lbm_handle *h, double rho0, double u0);
 
+/* f = feq(rho0, u0) with u0 a solenoidal random-phase field of shell
+ * spectrum E(s) = s^4 exp(-2 (s/kpeak)^2), s = 1..kmax, and per-component rms
+ * urms.  No reference counterpart: InitVelocityType::Decay is declared but
+ * not implemented (Options.h:27).  Specification:
+ *  - modes: integer k, |k_d| <= kmax (k_z = 0 in 2D); shell s(k) = nearest
+ *    integer to |k|; active when 1 <= s <= kmax; N_s active modes in shell s;
+ *  - the m-th active mode of the half space {k_x > 0} u {k_x = 0, k_y > 0} u
+ *    {k_x = k_y = 0, k_z > 0}, in ascending (k_x, k_y, k_z) order, takes
+ *    SplitMix64(seed) draws 3m, 3m+1, 3m+2 as angles theta1, theta2, phi =
+ *    2 pi (r >> 11) 2^-53;
+ *  - 3D: a = axis of the smallest |k_d| (ties: lowest d), e1 = k x a/|k x a|,
+ *    e2 = k x e1/|k x e1|, uh = A_s (cos phi e^{i theta1} e1 +
+ *    sin phi e^{i theta2} e2); 2D: uh = A_s e^{i theta1} (-k_y, k_x)/|k|;
+ *    A_s = sqrt(E(s)/N_s); uh(-k) = conj(uh(k));
+ *  - u(x) = gamma sum_k uh(k) e^{2 pi i (k_x x/nx + k_y y/ny + k_z z/nz)},
+ *    gamma = urms sqrt(D / sum_s E(s)), so mean |u|^2 = D urms^2.
+ * Requires 1 <= kmax and 2 kmax < every extent (LBM_E_INVALID otherwise). */
+int lbm_init_spectral(lbm_handle *h, double rho0, double urms, int kmax,
+                      double kpeak, uint64_t seed);
+
 /* Populations as FP64 f_i (unshifted) on the host, [Q][N]. */
 int lbm_set_populations(lbm_handle *h, const double *f);
 int lbm_get_populations(lbm_handle *h, double *f);

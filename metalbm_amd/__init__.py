@@ -199,6 +199,12 @@ class Algorithm:
     def init_taylor_green(self, rho0: float, u0: float) -> None:
         self._check(lib().lbm_init_taylor_green(self._h, rho0, u0))
 
+    def init_spectral(self, rho0: float, urms: float, kmax: int = 16, kpeak: float = 4.0,
+                      seed: int = 42) -> None:
+        """f = feq(rho0, u0), u0 solenoidal with random phases and shell
+        spectrum s^4 exp(-2 (s/kpeak)^2), s = 1..kmax, per-component rms urms."""
+        self._check(lib().lbm_init_spectral(self._h, rho0, urms, kmax, kpeak, seed))
+
     # -- populations / fields --------------------------------------------------
     def set_populations(self, f) -> None:
         a = _f64(f, (self.Q, self.N))

@@ -17,7 +17,7 @@ LBM_OK, LBM_E_INVALID, LBM_E_HIP, LBM_E_NOMEM, LBM_E_STATE = 0, -1, -2, -3, -4
 EXPORTS = (
     "lbm_abi_version", "lbm_config_default", "lbm_create", "lbm_destroy",
     "lbm_last_error", "lbm_sizes", "lbm_init_equilibrium", "lbm_init_noise",
-    "lbm_init_taylor_green", "lbm_set_populations", "lbm_get_populations",
+    "lbm_init_taylor_green", "lbm_init_spectral", "lbm_set_populations", "lbm_get_populations",
     "lbm_set_alpha", "lbm_get_alpha", "lbm_step", "lbm_get_fields", "lbm_reduce",
     "lbm_get_elbm_stats", "lbm_reset_elbm_stats", "lbm_set_stream",
     "lbm_synchronize", "lbm_device_populations", "lbm_steps_done",
@@ -83,6 +83,8 @@ def lib() -> ctypes.CDLL:
     L.lbm_init_equilibrium.argtypes = [H, dp, dp]
     L.lbm_init_noise.argtypes = [H, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]
     L.lbm_init_taylor_green.argtypes = [H, ctypes.c_double, ctypes.c_double]
+    L.lbm_init_spectral.argtypes = [H, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_uint64]
     L.lbm_set_populations.argtypes = [H, dp]
     L.lbm_get_populations.argtypes = [H, dp]
     L.lbm_set_alpha.argtypes = [H, dp]

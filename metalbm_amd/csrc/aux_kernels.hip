@@ -102,6 +102,81 @@ __global__ void k_taylor_green(double* u, int nx, int ny, int nz, int D,
   if (D == 3) u[(size_t)2 * N + c] = 0.0;
 }
 
+// Spectral initialisation: three separable inverse-DFT passes over a
+// Hermitian mode cube uh[M][M][Mz][D] (M = 2K+1, Mz = 2Kz+1), each pass a
+// sum in ascending wavenumber with twiddles e^{2 pi i t/n} from cos/sin
+// tables, in the operation order of the FP64 oracle.
+__device__ __forceinline__ int twiddle(int k, int p, int n) {
+  return ((k * p) % n + n) % n;
+}
+
+__global__ void k_spec_z(const double* ur, const double* ui, int M, int Mz, int Kz,
+                         int D, int nz, const double* tc, const double* ts,
+                         double* t1r, double* t1i) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * M * nz) return;
+  const int ixy = i / nz, z = i - ixy * nz;
+  for (int d = 0; d < D; ++d) {
+    double re = 0.0, im = 0.0;
+    for (int iz = 0; iz < Mz; ++iz) {
+      const int t = twiddle(iz - Kz, z, nz);
+      const double c = tc[t], sv = ts[t];
+      const size_t p = ((size_t)ixy * Mz + iz) * D + d;
+      const double a = ur[p], b = ui[p];
+      const double tr = a * c - b * sv;
+      const double ti = a * sv + b * c;
+      re += tr;
+      im += ti;
+    }
+    const size_t o = ((size_t)ixy * nz + z) * D + d;
+    t1r[o] = re;
+    t1i[o] = im;
+  }
+}
+
+__global__ void k_spec_y(const double* t1r, const double* t1i, int M, int K, int D,
+                         int ny, int nz, const double* tc, const double* ts,
+                         double* t2r, double* t2i) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * ny * nz) return;
+  const int z = i % nz, y = (i / nz) % ny, ix = i / (nz * ny);
+  for (int d = 0; d < D; ++d) {
+    double re = 0.0, im = 0.0;
+    for (int iy = 0; iy < M; ++iy) {
+      const int t = twiddle(iy - K, y, ny);
+      const double c = tc[t], sv = ts[t];
+      const size_t p = (((size_t)ix * M + iy) * nz + z) * D + d;
+      const double a = t1r[p], b = t1i[p];
+      const double tr = a * c - b * sv;
+      const double ti = a * sv + b * c;
+      re += tr;
+      im += ti;
+    }
+    const size_t o = (((size_t)ix * ny + y) * nz + z) * D + d;
+    t2r[o] = re;
+    t2i[o] = im;
+  }
+}
+
+__global__ void k_spec_x(const double* t2r, const double* t2i, int M, int K, int D,
+                         int nx, int ny, int nz, const double* tc, const double* ts,
+                         double gamma, double* u) {
+  const int N = nx * ny * nz;
+  const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c0 >= N) return;
+  const int z = c0 % nz, y = (c0 / nz) % ny, x = c0 / (nz * ny);
+  for (int d = 0; d < D; ++d) {
+    double re = 0.0;
+    for (int ix = 0; ix < M; ++ix) {
+      const int t = twiddle(ix - K, x, nx);
+      const size_t p = (((size_t)ix * ny + y) * nz + z) * D + d;
+      const double tr = t2r[p] * tc[t] - t2i[p] * ts[t];
+      re += tr;
+    }
+    u[(size_t)d * N + c0] = gamma * re;
+  }
+}
+
 template <class S>
 __global__ void k_pop_from_f64(S* dst, const double* src, int N, double w,
                                double scale) {
@@ -238,6 +313,23 @@ hipError_t launch_taylor_green(double* u, int nx, int ny, int nz, int D,
   const int N = nx * ny * nz;
   hipLaunchKernelGGL(k_taylor_green, dim3((N + 255) / 256), dim3(256), 0, st, u,
                      nx, ny, nz, D, u0, sx, cx, sy, cy, cz);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectral(const double* ur, const double* ui, int K, int D,
+                           int nx, int ny, int nz, const double* tab, double gamma,
+                           double* t1r, double* t1i, double* t2r, double* t2i,
+                           double* u, hipStream_t st) {
+  const int M = 2 * K + 1, Kz = D == 3 ? K : 0, Mz = 2 * Kz + 1;
+  const double *cx = tab, *sx = cx + nx, *cy = sx + nx, *sy = cy + ny,
+               *cz = sy + ny, *sz = cz + nz;
+  const int n1 = M * M * nz, n2 = M * ny * nz, n3 = nx * ny * nz;
+  hipLaunchKernelGGL(k_spec_z, dim3((n1 + 255) / 256), dim3(256), 0, st, ur, ui, M,
+                     Mz, Kz, D, nz, cz, sz, t1r, t1i);
+  hipLaunchKernelGGL(k_spec_y, dim3((n2 + 255) / 256), dim3(256), 0, st, t1r, t1i,
+                     M, K, D, ny, nz, cy, sy, t2r, t2i);
+  hipLaunchKernelGGL(k_spec_x, dim3((n3 + 255) / 256), dim3(256), 0, st, t2r, t2i,
+                     M, K, D, nx, ny, nz, cx, sx, gamma, u);
   return hipGetLastError();
 }
 

@@ -25,6 +25,15 @@ hipError_t launch_taylor_green(double* u, int nx, int ny, int nz, int D,
                                const double* sy, const double* cy,
                                const double* cz, hipStream_t st);
 
+// Spectral field u[D][N] = gamma Re sum_k uh(k) e^{2 pi i k.x/n} from the
+// Hermitian mode cube (ur, ui)[2K+1][2K+1][2Kz+1][D] (Kz = K in 3D, 0 in 2D)
+// by three separable passes.  tab = cos/sin tables per axis: cx[nx] sx[nx]
+// cy[ny] sy[ny] cz[nz] sz[nz].  t1: (2K+1)^2 nz D, t2: (2K+1) ny nz D doubles.
+hipError_t launch_spectral(const double* ur, const double* ui, int K, int D,
+                           int nx, int ny, int nz, const double* tab, double gamma,
+                           double* t1r, double* t1i, double* t2r, double* t2i,
+                           double* u, hipStream_t st);
+
 // One population between FP64 (unshifted) and storage format.
 hipError_t launch_pop_from_f64(int storage, void* dst, const double* src,
                                int N, double w, double scale, hipStream_t st);

@@ -67,6 +67,8 @@ struct lbm_handle {
   float* mlp = nullptr;
   float* mlp_m = nullptr;   // Adam first moments
   float* mlp_v = nullptr;   // Adam second moments
+  void* mlp_packed = nullptr;  // inference image of mlp (see mlp_packed_bytes)
+  bool mlp_dirty = true;    // mlp changed since the last pack
   int adam_t = 0;
   bool mlp_ready = false;   // normalisation set (trained or loaded)
   bool mlp_has_params = false;  // initialised or loaded (else all zero)
@@ -259,6 +261,8 @@ int build_force_tables(lbm_handle* h) {
   return LBM_OK;
 }
 
+int pack_mlp(lbm_handle* h);  // learned-alpha section below
+
 int ensure_staging(lbm_handle* h) {
   if (!h->staging)
     HIP_TRY(h, hipMalloc(&h->staging, (size_t)h->N * (1 + h->D) * sizeof(double)));
@@ -274,7 +278,7 @@ int ensure_fields(lbm_handle* h) {
 void release(lbm_handle* h) {
   for (void* p : {h->f[0], h->f[1], h->alpha, (void*)h->fields,
                   (void*)h->staging, (void*)h->reduce_buf, h->ftab[0],
-                  h->ftab[1], h->ftab[2], (void*)h->stats, (void*)h->mlp,
+                  h->ftab[1], h->ftab[2], (void*)h->stats, (void*)h->mlp, h->mlp_packed,
                   (void*)h->mlp_m, (void*)h->mlp_v, (void*)h->log_x,
                   (void*)h->log_alpha, (void*)h->log_count})
     if (p) (void)hipFree(p);
@@ -303,7 +307,7 @@ StepArgs make_args(const lbm_handle* h) {
   a.stats = h->stats;
   a.alpha_mode = h->alpha_mode;
   a.residual_tol = h->residual_tol;
-  a.mlp = h->mlp;
+  a.mlp_packed = h->mlp_packed;
   if (h->log_on) {
     a.log_x = h->log_x;
     a.log_alpha = h->log_alpha;
@@ -508,6 +512,165 @@ int lbm_init_taylor_green(lbm_handle* h, double rho0, double u0) {
   return LBM_OK;
 }
 
+namespace {
+
+int shell_of(int k2) {
+  int s = (int)std::sqrt((double)k2);
+  while ((2 * s + 1) * (2 * s + 1) <= 4 * k2) ++s;
+  while (s > 0 && (2 * s - 1) * (2 * s - 1) > 4 * k2) --s;
+  return s;
+}
+
+void cross3(const double* a, const double* b, double* r) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Hermitian mode cube of the spectral initialiser (specification in
+// lbm_init_spectral's declaration).  Returns gamma.
+double spectral_modes(int D, int K, double urms, double kpeak, uint64_t seed,
+                      std::vector<double>& ur, std::vector<double>& ui) {
+  const int M = 2 * K + 1, Kz = D == 3 ? K : 0, Mz = 2 * Kz + 1;
+  const size_t nm = (size_t)M * M * Mz;
+  auto active_half = [&](int kx, int ky, int kz, int& s) {
+    s = shell_of(kx * kx + ky * ky + kz * kz);
+    const bool half = kx > 0 || (kx == 0 && (ky > 0 || (ky == 0 && kz > 0)));
+    return std::make_pair(s >= 1 && s <= K, half);
+  };
+  std::vector<int> cnt(K + 1, 0);
+  size_t nh = 0;
+  for (int ix = 0; ix < M; ++ix)
+    for (int iy = 0; iy < M; ++iy)
+      for (int iz = 0; iz < Mz; ++iz) {
+        int s;
+        auto [act, half] = active_half(ix - K, iy - K, iz - Kz, s);
+        if (act) cnt[s]++;
+        if (act && half) nh++;
+      }
+  std::vector<double> amp(K + 1, 0.0);
+  double esum = 0.0;
+  for (int s = 1; s <= K; ++s) {
+    const double r = s / kpeak;
+    const double e = (double)s * s * s * s * std::exp(-2.0 * r * r);
+    esum += e;
+    amp[s] = cnt[s] > 0 ? std::sqrt(e / cnt[s]) : 0.0;
+  }
+  // Angles, then their cosines and sines in separate loops, so that no
+  // compiler fuses a sin/cos pair into one sincos call (rounded differently).
+  std::vector<double> ang(3 * nh), cs(3 * nh), sn(3 * nh);
+  for (size_t j = 0; j < 3 * nh; ++j) {
+    uint64_t z = seed + (j + 1) * 0x9E3779B97F4A7C15ULL;  // SplitMix64 draw j
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    ang[j] = 2.0 * M_PI * ((double)(z >> 11) * 0x1.0p-53);
+  }
+  for (size_t j = 0; j < 3 * nh; ++j) cs[j] = std::cos(ang[j]);
+  for (size_t j = 0; j < 3 * nh; ++j) sn[j] = std::sin(ang[j]);
+  ur.assign(nm * D, 0.0);
+  ui.assign(nm * D, 0.0);
+  size_t m = 0;
+  for (int ix = 0; ix < M; ++ix)
+    for (int iy = 0; iy < M; ++iy)
+      for (int iz = 0; iz < Mz; ++iz) {
+        const int kx = ix - K, ky = iy - K, kz = iz - Kz;
+        int s;
+        auto [act, half] = active_half(kx, ky, kz, s);
+        if (!(act && half)) continue;
+        const double c1 = cs[3 * m], s1 = sn[3 * m];
+        const double c2 = cs[3 * m + 1], s2 = sn[3 * m + 1];
+        const double cp = cs[3 * m + 2], sp = sn[3 * m + 2];
+        ++m;
+        double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+        if (D == 3) {
+          const double k[3] = {(double)kx, (double)ky, (double)kz};
+          int a = 0;
+          if (std::fabs(k[1]) < std::fabs(k[a])) a = 1;
+          if (std::fabs(k[2]) < std::fabs(k[a])) a = 2;
+          double ea[3] = {0, 0, 0}, e1[3], e2[3], w[3];
+          ea[a] = 1.0;
+          cross3(k, ea, w);
+          double nw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+          for (int d = 0; d < 3; ++d) e1[d] = w[d] / nw;
+          cross3(k, e1, w);
+          nw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+          for (int d = 0; d < 3; ++d) e2[d] = w[d] / nw;
+          for (int d = 0; d < 3; ++d) {
+            re[d] = amp[s] * (cp * c1 * e1[d] + sp * c2 * e2[d]);
+            im[d] = amp[s] * (cp * s1 * e1[d] + sp * s2 * e2[d]);
+          }
+        } else {
+          const double nk = std::sqrt((double)(kx * kx + ky * ky));
+          const double e1[2] = {-ky / nk, kx / nk};
+          for (int d = 0; d < 2; ++d) {
+            re[d] = amp[s] * (c1 * e1[d]);
+            im[d] = amp[s] * (s1 * e1[d]);
+          }
+        }
+        const size_t p = ((size_t)ix * M + iy) * Mz + iz;
+        const size_t q = ((size_t)(M - 1 - ix) * M + (M - 1 - iy)) * Mz + (Mz - 1 - iz);
+        for (int d = 0; d < D; ++d) {
+          ur[p * D + d] = re[d];
+          ui[p * D + d] = im[d];
+          ur[q * D + d] = re[d];
+          ui[q * D + d] = -im[d];
+        }
+      }
+  return urms * std::sqrt((double)D / esum);
+}
+
+}  // namespace
+
+int lbm_init_spectral(lbm_handle* h, double rho0, double urms, int kmax,
+                      double kpeak, uint64_t seed) {
+  if (!h) return LBM_E_INVALID;
+  const lbm_config& c = h->cfg;
+  const int n[3] = {c.nx, c.ny, c.nz};
+  if (kmax < 1) return fail(h, LBM_E_INVALID, "spectral init: kmax must be >= 1");
+  for (int d = 0; d < h->D; ++d)
+    if (2 * kmax >= n[d])
+      return fail(h, LBM_E_INVALID, "spectral init: 2 kmax must be below every extent");
+  if (!(kpeak > 0.0)) return fail(h, LBM_E_INVALID, "spectral init: kpeak must be > 0");
+  if (int rc = ensure_staging(h)) return rc;
+  std::vector<double> ur, ui;
+  const double gamma = spectral_modes(h->D, kmax, urms, kpeak, seed, ur, ui);
+  std::vector<double> tab;
+  const double two_pi = 2.0 * M_PI;
+  for (int d = 0; d < 3; ++d) {  // cos table, then sin table, per axis
+    for (int i = 0; i < n[d]; ++i) tab.push_back(std::cos(two_pi * i / n[d]));
+    for (int i = 0; i < n[d]; ++i) tab.push_back(std::sin(two_pi * i / n[d]));
+  }
+  const int M = 2 * kmax + 1;
+  const size_t nt1 = (size_t)M * M * c.nz * h->D, nt2 = (size_t)M * c.ny * c.nz * h->D;
+  const size_t total = 2 * ur.size() + tab.size() + 2 * nt1 + 2 * nt2;
+  double* buf = nullptr;
+  HIP_TRY(h, hipMalloc(&buf, total * sizeof(double)));
+  double* dur = buf;
+  double* dui = dur + ur.size();
+  double* dtab = dui + ui.size();
+  double* t1r = dtab + tab.size();
+  double* t1i = t1r + nt1;
+  double* t2r = t1i + nt1;
+  double* t2i = t2r + nt2;
+  hipError_t e = hipMemcpy(dur, ur.data(), ur.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dui, ui.data(), ui.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+  double* du = h->staging + h->N;
+  if (e == hipSuccess)
+    e = launch_spectral(dur, dui, kmax, h->D, c.nx, c.ny, c.nz, dtab, gamma, t1r, t1i,
+                        t2r, t2i, du, h->stream);
+  if (e == hipSuccess)
+    e = launch_init_equilibrium(c.lattice, c.equilibrium, c.storage, c.shifted,
+                                h->f[h->cur], nullptr, rho0, du, h->N, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return hip_fail(h, e, "spectral initialisation");
+  return LBM_OK;
+}
+
 int lbm_set_populations(lbm_handle* h, const double* f) {
   if (!h || !f) return h ? fail(h, LBM_E_INVALID, "null array") : LBM_E_INVALID;
   if (int rc = ensure_staging(h)) return rc;
@@ -575,6 +738,8 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
     return fail(h, LBM_E_STATE,
                 "learned alpha needs a network: lbm_mlp_train or lbm_mlp_set_params + "
                 "lbm_mlp_set_normalization");
+  if (h->cfg.collision == LBM_ELBM && h->alpha_mode == LBM_ALPHA_LEARNED)
+    if (int rc = pack_mlp(h)) return rc;
   StepArgs a = make_args(h);
   // Vector kernel: one wavefront per 64 V-cell segment, 4 per workgroup.
   const int grid = h->vec ? (h->rows * h->tiles_per_row + 3) / 4
@@ -706,9 +871,21 @@ int ensure_mlp(lbm_handle* h) {
     HIP_TRY(h, hipMalloc(&h->mlp, sizeof(float) * h->mlp_nb));
     HIP_TRY(h, hipMalloc(&h->mlp_m, sizeof(float) * h->mlp_np));
     HIP_TRY(h, hipMalloc(&h->mlp_v, sizeof(float) * h->mlp_np));
+    HIP_TRY(h, hipMalloc(&h->mlp_packed, mlp_packed_bytes(h->cfg.lattice)));
+    h->mlp_dirty = true;
     HIP_TRY(h, hipMemset(h->mlp, 0, sizeof(float) * h->mlp_nb));
     HIP_TRY(h, hipMemset(h->mlp_m, 0, sizeof(float) * h->mlp_np));
     HIP_TRY(h, hipMemset(h->mlp_v, 0, sizeof(float) * h->mlp_np));
+  }
+  return LBM_OK;
+}
+
+// Refreshes the packed inference image after any change of the parameters.
+int pack_mlp(lbm_handle* h) {
+  if (int rc = ensure_mlp(h)) return rc;
+  if (h->mlp_dirty) {
+    HIP_TRY(h, launch_mlp_pack(h->cfg.lattice, h->mlp, h->mlp_packed, h->stream));
+    h->mlp_dirty = false;
   }
   return LBM_OK;
 }
@@ -751,6 +928,7 @@ int lbm_mlp_init(lbm_handle* h, uint64_t seed) {
   p[h->mlp_np + 2 * nin] = 2.0f;
   p[h->mlp_np + 2 * nin + 1] = 1.0f;
   HIP_TRY(h, hipMemcpy(h->mlp, p.data(), sizeof(float) * h->mlp_nb, hipMemcpyHostToDevice));
+  h->mlp_dirty = true;
   HIP_TRY(h, hipMemset(h->mlp_m, 0, sizeof(float) * h->mlp_np));
   HIP_TRY(h, hipMemset(h->mlp_v, 0, sizeof(float) * h->mlp_np));
   h->adam_t = 0;
@@ -762,6 +940,7 @@ int lbm_mlp_set_params(lbm_handle* h, const float* params) {
   if (!params) return fail(h, LBM_E_INVALID, "null params");
   if (int rc = ensure_mlp(h)) return rc;
   HIP_TRY(h, hipMemcpy(h->mlp, params, sizeof(float) * h->mlp_np, hipMemcpyHostToDevice));
+  h->mlp_dirty = true;
   h->mlp_has_params = true;
   return LBM_OK;
 }
@@ -786,6 +965,7 @@ int lbm_mlp_set_normalization(lbm_handle* h, const float* in_mean, const float* 
   t[2 * h->mlp_nin + 1] = (float)out_scale;
   HIP_TRY(h, hipMemcpy(h->mlp + h->mlp_np, t.data(), sizeof(float) * t.size(),
                        hipMemcpyHostToDevice));
+  h->mlp_dirty = true;
   h->mlp_ready = true;
   return LBM_OK;
 }
@@ -868,7 +1048,9 @@ int lbm_mlp_predict(lbm_handle* h, const float* x, int n, double* alpha) {
   if (e == hipSuccess) e = hipMalloc(&da, sizeof(double) * (size_t)n);
   if (e == hipSuccess)
     e = hipMemcpy(dx, x, sizeof(float) * (size_t)n * h->mlp_nin, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_mlp_forward(h->cfg.lattice, h->mlp, dx, n, da, h->stream);
+  if (e == hipSuccess && pack_mlp(h) != LBM_OK) e = hipErrorUnknown;
+  if (e == hipSuccess)
+    e = launch_mlp_forward(h->cfg.lattice, h->mlp_packed, dx, n, da, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess) e = hipMemcpy(alpha, da, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost);
   if (dx) (void)hipFree(dx);
@@ -1008,6 +1190,7 @@ int lbm_mlp_train(lbm_handle* h, int iterations, int batch, double lr, uint64_t 
     h->adam_t += 1;
     const float c1 = (float)(1.0 - std::pow(0.9, h->adam_t));
     const float c2 = (float)(1.0 - std::pow(0.999, h->adam_t));
+    h->mlp_dirty = true;
     HIP_TRY(h, launch_mlp_adam(h->mlp, b.g, h->mlp_m, h->mlp_v, np, (float)lr, b1, b2,
                                eps, c1, c2, h->stream));
     if ((it == 0 && loss_first) || (it == iterations - 1 && loss_last)) {

@@ -117,25 +117,43 @@ __global__ void __launch_bounds__(BLOCK) k_bgk(const StepArgs a, int FSX) {
 
 // ---- entropic collision ------------------------------------------------------
 
+// FP64 view of one cell's pulled populations f and non-equilibrium parts n,
+// held in the compute type C (f stored as f - w when SH).  Values are
+// converted at each use, so the alpha solve keeps no FP64 copies of them
+// (exact for C = double: the view then returns the stored values).
+template <class L, class C, bool SH>
+struct Fp64View {
+  const C* fs;
+  const C* ns;
+  template <int i>
+  __device__ __forceinline__ double f() const {
+    return SH ? double(fs[i]) + L::W[i] : double(fs[i]);
+  }
+  template <int i>
+  __device__ __forceinline__ double n() const {
+    return double(ns[i]);
+  }
+};
+
 // ELBM alpha (Collision.h:351-375), first part: the deviation test
 // (Collision.h:284-303) and alphaMax (Collision.h:305-326).  Returns true when
 // the cell needs the root solve; otherwise alpha and branch are final
-// (0: deviation small -> 2; 1: alphaMax < 2 -> 0.95 alphaMax).  f and n
-// (= fneq) are the pulled populations and their non-equilibrium parts in
-// FP64.  REFDIV keeps the reference's divisions in the deviation test.
-template <class L, bool REFDIV>
-__device__ __forceinline__ bool elbm_prelude(const double* f, const double* n,
-                                             double& alpha, double& amax,
-                                             int& branch) {
+// (0: deviation small -> 2; 1: alphaMax < 2 -> 0.95 alphaMax).  p views
+// the pulled populations and their non-equilibrium parts in FP64.  REFDIV
+// keeps the reference's divisions in the deviation test.
+template <class L, bool REFDIV, class V>
+__device__ __forceinline__ bool elbm_prelude(const V& p, double& alpha,
+                                             double& amax, int& branch) {
   constexpr int Q = L::dimQ;
   bool small = true;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
+    const double fi = p.template f<i>(), ni = p.template n<i>();
     if constexpr (REFDIV) {
-      const double dev = fabs(n[i]) / f[i];
+      const double dev = fabs(ni) / fi;
       if (dev > 1.0e-3) small = false;
     } else {
-      if (!(fabs(n[i]) <= 1.0e-3 * f[i]) && f[i] > 0.0) small = false;
+      if (!(fabs(ni) <= 1.0e-3 * fi) && fi > 0.0) small = false;
     }
   });
   if (small) {
@@ -146,8 +164,9 @@ __device__ __forceinline__ bool elbm_prelude(const double* f, const double* n,
   amax = 2.5;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    if (n[i] > 0) {
-      const double t = fabs(f[i]) / n[i];
+    const double fi = p.template f<i>(), ni = p.template n<i>();
+    if (ni > 0) {
+      const double t = fabs(fi) / ni;
       if (t < amax) amax = t;
     }
   });
@@ -164,16 +183,15 @@ __device__ __forceinline__ bool elbm_prelude(const double* f, const double* n,
 // the iterate before the last step is returned, alpha = 2 when it fails or
 // leaves (1, alphaMax).  G and G' share one log per term; their values equal
 // the reference's separate loops (EntropicStep.h:31-62).
-template <class L>
-__device__ __forceinline__ double newton_reference(const double* f,
-                                                   const double* n,
-                                                   double guess, double amax,
-                                                   int& branch, int& iterations) {
+template <class L, class V>
+__device__ __forceinline__ double newton_reference(const V& p, double guess,
+                                                   double amax, int& branch,
+                                                   int& iterations) {
   constexpr int Q = L::dimQ;
   double A[Q];
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    A[i] = log(f[i] / L::W[i]);
+    A[i] = log(p.template f<i>() / L::W[i]);
   });
   double x = guess, step = 0.0;
   bool ok = false;
@@ -184,10 +202,11 @@ __device__ __forceinline__ double newton_reference(const double* f,
     double g = 0.0, dg = 0.0;
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      const double fm = f[i] - x * n[i];
+      const double fi = p.template f<i>(), ni = p.template n<i>();
+      const double fm = fi - x * ni;
       const double lg = log(fm / L::W[i]);
-      g += f[i] * A[i] - fm * lg;
-      dg += n[i] * (1 + lg);
+      g += fi * A[i] - fm * lg;
+      dg += ni * (1 + lg);
     });
     step = g / dg;
     if (fabs(step) <= 1e-8) {
@@ -219,7 +238,8 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // inference runs with every lane present: (1) pull, moments, fneq and the
 // alpha prelude per cell; (2) MODE 1 only: the network predicts alpha for
 // every lane's cell at once (its barriers are reached by all threads); (3) the
-// alpha solve, the collision and the stores per cell.
+// alpha solve, the collision and the stores per cell.  The network is staged
+// before phase 1 and published by one workgroup barrier after it.
 template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
   constexpr int Q = L::dimQ;
@@ -229,12 +249,8 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
   extern __shared__ __align__(16) char s_lds[];
   if (a.stats && threadIdx.x < 6) s_cnt[threadIdx.x] = 0u;
   static_assert(MODE != 1 || Shape::NIN > 0, "no alpha network for this lattice");
-  if constexpr (MODE == 1) {
-    {  // stage the network once per workgroup
-      stage_mfma_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp, s_lds);
-      __syncthreads();
-    }
-  }
+  if constexpr (MODE == 1)  // stage the packed network; the barrier follows phase 1
+    copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, s_lds);
   const CellPos c = cell_of_thread<L, BLOCK>(a);
   int branch = -1, its = 0;
   bool first = false, solve = false;
@@ -275,15 +291,10 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
       });
     }
     // The alpha solve runs in FP64 whatever the compute type.
-    double fd[Q], nd[Q];
-    static_for<Q>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      fd[i] = SH ? double(f[i]) + L::W[i] : double(f[i]);
-      nd[i] = double(fneq[i]);
-    });
-    solve = elbm_prelude<L, REF>(fd, nd, alpha, amax, branch);
+    solve = elbm_prelude<L, REF>(Fp64View<L, C, SH>{f, fneq}, alpha, amax, branch);
   }
   // ---- phase 2 ----
+  if constexpr (MODE == 1) __syncthreads();  // the staged network is visible
   double start = 2.0;
   float m[Shape::NIN > 0 ? Shape::NIN : 1];
   if constexpr (Shape::NIN > 0) {
@@ -295,25 +306,19 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
       for (int k = 0; k < Shape::NIN; ++k) m[k] = 0.0f;
     }
     if constexpr (MODE == 1)
-      start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2>(s_lds, m);
+      start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2, typename Shape::Op>(s_lds, m);
   }
   // ---- phase 3 ----
   if (c.active) {
     S* __restrict__ dst = static_cast<S*>(a.dst);
     C* __restrict__ astate = static_cast<C*>(a.alpha_state);
     if (solve) {
-      double fd[Q], nd[Q];
-      static_for<Q>([&](auto ic) {
-        constexpr int i = decltype(ic)::value;
-        fd[i] = SH ? double(f[i]) + L::W[i] : double(f[i]);
-        nd[i] = double(fneq[i]);
-      });
+      const Fp64View<L, C, SH> p{f, fneq};
       if constexpr (warm) start = double(astate[cell]);
       if constexpr (MODE == 0)
-        alpha = newton_reference<L>(fd, nd, start, amax, branch, its);
+        alpha = newton_reference<L>(p, start, amax, branch, its);
       else
-        alpha = residual_newton<L>(fd, nd, amax, start, a.residual_tol, branch,
-                                   its, first);
+        alpha = residual_newton<L>(p, amax, start, a.residual_tol, branch, its, first);
       if constexpr (Shape::NIN > 0) {
         if (a.log_cap > 0 && branch == 2 &&
             sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {

@@ -19,17 +19,23 @@
 
 namespace lbm {
 
+// Network shape per lattice and the matrix-core operand type of its
+// inference (D2Q9: f16, 11-bit significand; D3Q27: bf16 weights as specified
+// for the 27-64-64-1 network).  Master weights and training stay FP32.
 template <class L>
 struct MlpShape {
   static constexpr int NIN = 0, H1 = 0, H2 = 0;
+  using Op = _Float16;
 };
 template <>
 struct MlpShape<Lattice<LatticeType::D2Q9>> {
   static constexpr int NIN = 9, H1 = 32, H2 = 32;
+  using Op = _Float16;
 };
 template <>
 struct MlpShape<Lattice<LatticeType::D3Q27>> {
   static constexpr int NIN = 27, H1 = 64, H2 = 64;
+  using Op = __bf16;
 };
 
 template <int NIN, int H1, int H2>
@@ -76,14 +82,14 @@ __device__ __forceinline__ void nonequilibrium_moments(const T* fneq, T rrho,
 // ---- wavefront inference on the matrix cores --------------------------------
 //
 // The fused kernel runs the network for all 64 cells of a wavefront with
-// v_mfma_f32_16x16x16f16 (f16 operands, FP32 accumulation).  Cells are the N
+// v_mfma_f32_16x16x16 (f16 or bf16 operands, FP32 accumulation).  Cells are the N
 // dimension, so a layer's accumulator (lane l: cell l%16 of the tile, rows
 // 4(l/16)+j) is already the next layer's B operand (lane l: column l%16,
 // k = 4(l/16)+j): H^T = relu(W . X^T + b) chains without a transpose.
 // LDS layout (bytes), staged once per workgroup from the FP32 block:
-//   W1h[H1][KP] W2h[H2][H1] W3h[16][H2] (f16, zero padded; W3h row 0 = W3)
+//   W1h[H1][KP] W2h[H2][H1] W3h[16][H2] (operand type, zero padded; W3h row 0 = W3)
 //   b1[H1] b2[H2] b3 mean[NIN] scale[NIN] out[2] (f32)
-//   X[4 waves][64][KP] (f16 standardised features)  Y[4 waves][64] (f32)
+//   X[4 waves][64][KP] (standardised features)  Y[4 waves][64] (f32)
 template <int NIN, int H1, int H2>
 struct MfmaLayout {
   static constexpr int KP = NIN <= 16 ? 16 : 32;
@@ -98,37 +104,79 @@ struct MfmaLayout {
   static constexpr int BYTES = Y + 4 * 64 * 4;
 };
 
-typedef _Float16 mfma_h4 __attribute__((ext_vector_type(4)));
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
-// Cooperative staging of the FP32 parameter block into the LDS layout.
-template <int NIN, int H1, int H2>
+// One 16x16x16 matrix-core product, D = A B + C, for operand type T.
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<_Float16> {
+  typedef _Float16 v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ mfma_f4 mac(v4 a, v4 b, mfma_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<__bf16> {
+  typedef __bf16 v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ mfma_f4 mac(v4 a, v4 b, mfma_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+};
+
+// Writes the FP32 parameter block in the LDS layout (bytes [0, X)), the
+// weights rounded to the operand type T (round to nearest even).  Run once per
+// parameter change into a global "packed" buffer; kernels then stage it with
+// copy_packed_network.  Cooperative over the threads of one workgroup.
+template <int NIN, int H1, int H2, class T>
 __device__ __forceinline__ void stage_mfma_network(const float* __restrict__ P,
                                                    char* lds) {
   using Dm = MlpDims<NIN, H1, H2>;
   using M = MfmaLayout<NIN, H1, H2>;
-  _Float16* w1 = reinterpret_cast<_Float16*>(lds + M::W1H);
-  _Float16* w2 = reinterpret_cast<_Float16*>(lds + M::W2H);
-  _Float16* w3 = reinterpret_cast<_Float16*>(lds + M::W3H);
+  T* w1 = reinterpret_cast<T*>(lds + M::W1H);
+  T* w2 = reinterpret_cast<T*>(lds + M::W2H);
+  T* w3 = reinterpret_cast<T*>(lds + M::W3H);
   float* fl = reinterpret_cast<float*>(lds + M::F32);
   for (int t = threadIdx.x; t < H1 * M::KP; t += blockDim.x) {
     const int r = t / M::KP, k = t - r * M::KP;
-    w1[t] = k < NIN ? _Float16(P[Dm::W1 + r * NIN + k]) : _Float16(0.0f);
+    w1[t] = k < NIN ? T(P[Dm::W1 + r * NIN + k]) : T(0.0f);
   }
-  for (int t = threadIdx.x; t < H2 * H1; t += blockDim.x) w2[t] = _Float16(P[Dm::W2 + t]);
+  for (int t = threadIdx.x; t < H2 * H1; t += blockDim.x) w2[t] = T(P[Dm::W2 + t]);
   for (int t = threadIdx.x; t < 16 * H2; t += blockDim.x)
-    w3[t] = t < H2 ? _Float16(P[Dm::W3 + t]) : _Float16(0.0f);
+    w3[t] = t < H2 ? T(P[Dm::W3 + t]) : T(0.0f);
   for (int t = threadIdx.x; t < H1; t += blockDim.x) fl[t] = P[Dm::B1 + t];
   for (int t = threadIdx.x; t < H2; t += blockDim.x) fl[H1 + t] = P[Dm::B2 + t];
   for (int t = threadIdx.x; t < 2 * NIN + 3; t += blockDim.x)
     fl[H1 + H2 + t] = P[Dm::B3 + t];  // b3, mean[NIN], scale[NIN], out[2]
 }
 
-// alpha_hat for every lane's cell (m: raw features; lanes without a cell pass
-// zeros).  Must be reached by all threads of the workgroup (two barriers).
+// Workgroup-cooperative copy of a packed network (M::X bytes, 16-aligned).
 template <int NIN, int H1, int H2>
+__device__ __forceinline__ void copy_packed_network(const void* __restrict__ src,
+                                                    char* lds) {
+  constexpr int n = MfmaLayout<NIN, H1, H2>::X / 16;
+  const uint4* s = static_cast<const uint4*>(src);
+  uint4* d = reinterpret_cast<uint4*>(lds);
+  for (int t = threadIdx.x; t < n; t += blockDim.x) d[t] = s[t];
+}
+
+// Orders one wavefront's LDS writes before its later LDS reads by other lanes
+// (the LDS executes a wavefront's operations in order; the fences stop the
+// compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// alpha_hat for every lane's cell (m: raw features; lanes without a cell pass
+// zeros), from the network staged in LDS (visible to the workgroup).  Each
+// wavefront uses only its own feature and output tiles, so it synchronises
+// with itself only; all 64 lanes of the wavefront must be active.
+template <int NIN, int H1, int H2, class T>
 __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
   using M = MfmaLayout<NIN, H1, H2>;
+  using V4 = typename Mfma<T>::v4;
   constexpr int KP = M::KP;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const float* fl = reinterpret_cast<const float*>(lds + M::F32);
@@ -137,59 +185,54 @@ __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
   const float b3 = fl[H1 + H2];
   const float* mean = fl + H1 + H2 + 1;
   const float* scale = mean + NIN;
-  _Float16* X = reinterpret_cast<_Float16*>(lds + M::X) + wv * 64 * KP;
+  T* X = reinterpret_cast<T*>(lds + M::X) + wv * 64 * KP;
   float* Y = reinterpret_cast<float*>(lds + M::Y) + wv * 64;
 #pragma unroll
   for (int k4 = 0; k4 < KP; k4 += 4) {
-    mfma_h4 z;
+    V4 z;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int k = k4 + j;
-      z[j] = k < NIN ? _Float16((m[k] - mean[k]) * scale[k]) : _Float16(0.0f);
+      z[j] = k < NIN ? T((m[k] - mean[k]) * scale[k]) : T(0.0f);
     }
-    *reinterpret_cast<mfma_h4*>(X + lane * KP + k4) = z;
+    *reinterpret_cast<V4*>(X + lane * KP + k4) = z;
   }
-  __syncthreads();
-  const _Float16* w1 = reinterpret_cast<const _Float16*>(lds + M::W1H);
-  const _Float16* w2 = reinterpret_cast<const _Float16*>(lds + M::W2H);
-  const _Float16* w3 = reinterpret_cast<const _Float16*>(lds + M::W3H);
+  wave_sync_lds();
+  const T* w1 = reinterpret_cast<const T*>(lds + M::W1H);
+  const T* w2 = reinterpret_cast<const T*>(lds + M::W2H);
+  const T* w3 = reinterpret_cast<const T*>(lds + M::W3H);
   const int col = lane & 15, kq = 4 * (lane >> 4);
-  auto frag = [](const _Float16* p) { return *reinterpret_cast<const mfma_h4*>(p); };
+  auto frag = [](const T* p) { return *reinterpret_cast<const V4*>(p); };
 #pragma unroll
   for (int t = 0; t < 4; ++t) {  // 16-cell tiles of the wavefront
-    mfma_h4 h1[H1 / 16];
+    V4 h1[H1 / 16];
 #pragma unroll
     for (int rt = 0; rt < H1 / 16; ++rt) {
       mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kb = 0; kb < KP / 16; ++kb)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
-            frag(w1 + (rt * 16 + col) * KP + kb * 16 + kq),
-            frag(X + (t * 16 + col) * KP + kb * 16 + kq), acc, 0, 0, 0);
+        acc = Mfma<T>::mac(frag(w1 + (rt * 16 + col) * KP + kb * 16 + kq),
+                           frag(X + (t * 16 + col) * KP + kb * 16 + kq), acc);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        h1[rt][j] = _Float16(fmaxf(acc[j] + b1[rt * 16 + kq + j], 0.0f));
+      for (int j = 0; j < 4; ++j) h1[rt][j] = T(fmaxf(acc[j] + b1[rt * 16 + kq + j], 0.0f));
     }
-    mfma_h4 h2[H2 / 16];
+    V4 h2[H2 / 16];
 #pragma unroll
     for (int rt = 0; rt < H2 / 16; ++rt) {
       mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kb = 0; kb < H1 / 16; ++kb)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
-            frag(w2 + (rt * 16 + col) * H1 + kb * 16 + kq), h1[kb], acc, 0, 0, 0);
+        acc = Mfma<T>::mac(frag(w2 + (rt * 16 + col) * H1 + kb * 16 + kq), h1[kb], acc);
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        h2[rt][j] = _Float16(fmaxf(acc[j] + b2[rt * 16 + kq + j], 0.0f));
+      for (int j = 0; j < 4; ++j) h2[rt][j] = T(fmaxf(acc[j] + b2[rt * 16 + kq + j], 0.0f));
     }
     mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kb = 0; kb < H2 / 16; ++kb)
-      acc = __builtin_amdgcn_mfma_f32_16x16x16f16(
-          frag(w3 + col * H2 + kb * 16 + kq), h2[kb], acc, 0, 0, 0);
+      acc = Mfma<T>::mac(frag(w3 + col * H2 + kb * 16 + kq), h2[kb], acc);
     if (lane < 16) Y[t * 16 + lane] = acc[0];  // row 0 = the output unit
   }
-  __syncthreads();
+  wave_sync_lds();
   const float y = Y[lane] + b3;
   return double(fl[H1 + H2 + 1 + 2 * NIN]) + double(fl[H1 + H2 + 2 + 2 * NIN]) * double(y);
 }
@@ -225,86 +268,94 @@ __device__ __forceinline__ double log1p_small(double y) {
   p = fma(p, y, 1.0);
   return p * y;
 }
-// 1/d for d near 1 (|1 - d| <= kSeriesMax) to full precision: hardware
-// reciprocal estimate plus two Newton-Raphson corrections.
-__device__ __forceinline__ double recip_near_one(double d) {
+// 1/d for d > 0 in the normal range: hardware reciprocal estimate plus two
+// Newton-Raphson corrections (each squares the relative error), a few ulp.
+__device__ __forceinline__ double recip_refined(double d) {
   double r = __builtin_amdgcn_rcp(d);
   r = fma(r, fma(-d, r, 1.0), r);
   return fma(r, fma(-d, r, 1.0), r);
 }
 
-// n / f for f > 0 in the normal range: reciprocal estimate, two
-// Newton-Raphson corrections and a final residual correction of the quotient
-// (error of a few ulp, well inside the solver's rounding allowance).
+// n / f for f > 0 in the normal range: refined reciprocal and a final
+// residual correction of the quotient (error of a few ulp, well inside the
+// solver's rounding allowance).
 __device__ __forceinline__ double div_refined(double n, double f) {
-  double r = __builtin_amdgcn_rcp(f);
-  r = fma(r, fma(-f, r, 1.0), r);
-  r = fma(r, fma(-f, r, 1.0), r);
+  const double r = recip_refined(f);
   const double q = n * r;
   return fma(r, fma(-f, q, n), q);
 }
 
-// G, G' and -G'' at alpha = a (t_i = a x_i), with log1p and 1/(1 - t) by
-// series and refined reciprocal when every |t_i| <= kSeriesMax, else by the
-// library.  The choice is made once per evaluation, not per term, so the two
-// variants are never both computed.
-template <class L>
-__device__ __forceinline__ void residual_terms(const double* f, const double* n,
-                                               const double* x, const double* A,
-                                               double a, double& g, double& dg,
-                                               double& h) {
+// log1p(y) for |y| <= kMidMax as 2 atanh(z), z = y / (2 + y), |z| <= 1/7:
+// 2 z sum_{k=0..9} z^{2k} / (2k + 1), truncation < 2 |z| (z^2)^10 / 21 / (1 - z^2)
+// < 2e-18 |z|.
+constexpr double kMidMax = 0.25;
+__device__ __forceinline__ double log1p_mid(double y) {
+  const double z = div_refined(y, 2.0 + y);
+  const double p = z * z;
+  double s = 1.0 / 19.0;
+  s = fma(s, p, 1.0 / 17.0);
+  s = fma(s, p, 1.0 / 15.0);
+  s = fma(s, p, 1.0 / 13.0);
+  s = fma(s, p, 1.0 / 11.0);
+  s = fma(s, p, 1.0 / 9.0);
+  s = fma(s, p, 1.0 / 7.0);
+  s = fma(s, p, 1.0 / 5.0);
+  s = fma(s, p, 1.0 / 3.0);
+  s = fma(s, p, 1.0);
+  return 2.0 * z * s;
+}
+
+// G, G' and -G'' at alpha = a from t_i = a x_i, with S = sum n_i A_i and
+// Sn = sum n_i precomputed (A_i = ln(f_i / w_i)):
+//   G = a S - sum (f_i - a n_i) log1p(-t_i),  G' = Sn + S + sum n_i log1p(-t_i),
+//   -G'' = sum n_i x_i / (1 - t_i)
+// (f - a n = f (1 - t), ln((f - a n)/w) = A + log1p(-t)).  log1p(-t) by the
+// degree-12 Taylor series when every |t_i| <= kSeriesMax, by the atanh
+// series when every |t_i| <= kMidMax, else by the library; the choice is made
+// once per evaluation, not per term.  1/(1 - t) by the refined reciprocal.
+template <class L, class V>
+__device__ __forceinline__ void residual_terms(const V& p, const double* x, double S,
+                                               double Sn, double a, double& g,
+                                               double& dg, double& h) {
   constexpr int Q = L::dimQ;
   double tmax = 0.0;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     tmax = fmax(tmax, fabs(a * x[i]));
   });
-  g = 0.0;
-  dg = 0.0;
+  double gs = 0.0, ds = 0.0;
   h = 0.0;
-  if (tmax <= kSeriesMax) {
+  auto terms = [&](auto log1p_of) {
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
+      const double ni = p.template n<i>();
       const double t = a * x[i];
-      const double l1 = log1p_small(-t);
-      g += f[i] * (t * A[i] - (1.0 - t) * l1);
-      dg += n[i] * (1.0 + A[i] + l1);
-      h += n[i] * x[i] * recip_near_one(1.0 - t);
+      const double l1 = log1p_of(-t);
+      gs += (p.template f<i>() - a * ni) * l1;
+      ds += ni * l1;
+      h += ni * x[i] * recip_refined(1.0 - t);
     });
-  } else {
-    static_for<Q>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      const double t = a * x[i];
-      const double l1 = log1p(-t);
-      g += f[i] * (t * A[i] - (1.0 - t) * l1);
-      dg += n[i] * (1.0 + A[i] + l1);
-      h += n[i] * x[i] / (1.0 - t);
-    });
-  }
+  };
+  if (tmax <= kSeriesMax)
+    terms([](double y) { return log1p_small(y); });
+  else if (tmax <= kMidMax)
+    terms([](double y) { return log1p_mid(y); });
+  else
+    terms([](double y) { return log1p(y); });
+  g = a * S - gs;
+  dg = Sn + S + ds;
 }
 
 // -G''(a) = sum n_i x_i / (1 - a x_i) >= 0, as in residual_terms.
-template <class L>
-__device__ __forceinline__ double residual_curvature(const double* n,
-                                                     const double* x, double a) {
+template <class L, class V>
+__device__ __forceinline__ double residual_curvature(const V& p, const double* x,
+                                                     double a) {
   constexpr int Q = L::dimQ;
-  double tmax = 0.0;
+  double s = 0.0;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    tmax = fmax(tmax, fabs(a * x[i]));
+    s += p.template n<i>() * x[i] * recip_refined(1.0 - a * x[i]);
   });
-  double s = 0.0;
-  if (tmax <= kSeriesMax) {
-    static_for<Q>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      s += n[i] * x[i] * recip_near_one(1.0 - a * x[i]);
-    });
-  } else {
-    static_for<Q>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      s += n[i] * x[i] / (1.0 - a * x[i]);
-    });
-  }
   return s;
 }
 
@@ -312,40 +363,47 @@ __device__ __forceinline__ double residual_curvature(const double* n,
 // 31-45): accepts alpha once |G| <= tol, or after a Newton step s whose
 // Taylor remainder bound 1/2 s^2 max|G''| (G'' is convex in alpha, so its
 // maximum over the step lies at an end point) is <= tol, with a relative
-// rounding allowance.  Returns 2 with branch 3 if an iterate leaves
-// (1, amax) or 50 evaluations do not converge.
-template <class L>
-__device__ __forceinline__ double residual_newton(const double* f,
-                                                  const double* n, double amax,
+// rounding allowance.  Every evaluated point lies in (1, amax), where all
+// logarithms are defined: a Newton step that would leave the interval is
+// replaced by the midpoint towards the violated end (G is concave with
+// G(0) = 0, so the iteration then approaches the root from the right).
+// Returns 2 with branch 3 when 50 evaluations do not converge or the step is
+// not finite.  p views f and fneq in FP64.
+template <class L, class V>
+__device__ __forceinline__ double residual_newton(const V& p, double amax,
                                                   double start, double tol,
                                                   int& branch, int& evals,
                                                   bool& first) {
   constexpr int Q = L::dimQ;
-  double x[Q], A[Q];
-  bool small_a = true;
+  double x[Q];
+  double ymax = 0.0;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    x[i] = div_refined(n[i], f[i]);
-    A[i] = fma(f[i], 1.0 / L::W[i], -1.0);  // f/w - 1
-    small_a = small_a && fabs(A[i]) <= kSeriesMax;
+    const double fi = p.template f<i>();
+    x[i] = div_refined(p.template n<i>(), fi);
+    ymax = fmax(ymax, fabs(fma(fi, 1.0 / L::W[i], -1.0)));
   });
-  if (small_a) {
+  double S = 0.0, Sn = 0.0;  // sum n ln(f/w), sum n
+  auto sums = [&](auto log1p_of) {
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      A[i] = log1p_small(A[i]);  // ln(f/w)
+      const double ni = p.template n<i>();
+      S += ni * log1p_of(fma(p.template f<i>(), 1.0 / L::W[i], -1.0));
+      Sn += ni;
     });
-  } else {
-    static_for<Q>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      A[i] = log1p(A[i]);
-    });
-  }
+  };
+  if (ymax <= kSeriesMax)
+    sums([](double y) { return log1p_small(y); });
+  else if (ymax <= kMidMax)
+    sums([](double y) { return log1p_mid(y); });
+  else
+    sums([](double y) { return log1p(y); });
   double a = (start > 1.0 && start < amax) ? start : 2.0;
   evals = 0;
   first = false;
   for (int it = 0; it < 50; ++it) {
     double g, dg, h;
-    residual_terms<L>(f, n, x, A, a, g, dg, h);
+    residual_terms<L>(p, x, S, Sn, a, g, dg, h);
     ++evals;
     if (fabs(g) <= tol) {
       branch = 2;
@@ -353,10 +411,14 @@ __device__ __forceinline__ double residual_newton(const double* f,
       return a;
     }
     const double s = g / dg;
+    if (!isfinite(s)) break;
     const double an = a - s;
-    if (!(an > 1.0 && an < amax)) break;
+    if (!(an > 1.0 && an < amax)) {
+      a = an >= amax ? 0.5 * (a + amax) : 0.5 * (a + 1.0);
+      continue;
+    }
     const double bound =
-        0.5 * s * s * fmax(h, residual_curvature<L>(n, x, an)) * (1.0 + 1e-6);
+        0.5 * s * s * fmax(h, residual_curvature<L>(p, x, an)) * (1.0 + 1e-6);
     if (bound <= tol) {
       branch = 2;
       first = it == 0;

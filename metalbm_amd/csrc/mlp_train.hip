@@ -87,17 +87,23 @@ k_mlp_adam(float* __restrict__ P, const float* __restrict__ g,
 
 // Batch inference with the step kernel's own matrix-core code (f16 operands,
 // FP32 accumulation), so predictions equal what the fused step starts from.
-template <int NIN, int H1, int H2>
+template <int NIN, int H1, int H2, class T>
+__global__ void __launch_bounds__(256) k_mlp_pack(const float* __restrict__ P,
+                                                  char* __restrict__ packed) {
+  stage_mfma_network<NIN, H1, H2, T>(P, packed);
+}
+
+template <int NIN, int H1, int H2, class T>
 __global__ void __launch_bounds__(256)
-k_mlp_forward(const float* __restrict__ P, const float* __restrict__ X, int n,
+k_mlp_forward(const void* __restrict__ packed, const float* __restrict__ X, int n,
               double* __restrict__ alpha) {
   extern __shared__ __align__(16) char s_lds[];
-  stage_mfma_network<NIN, H1, H2>(P, s_lds);
+  copy_packed_network<NIN, H1, H2>(packed, s_lds);
   __syncthreads();
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   float m[NIN];
   for (int k = 0; k < NIN; ++k) m[k] = s < n ? X[(size_t)s * NIN + k] : 0.0f;
-  const double y = mlp_alpha_wave<NIN, H1, H2>(s_lds, m);
+  const double y = mlp_alpha_wave<NIN, H1, H2, T>(s_lds, m);
   if (s < n) alpha[s] = y;
 }
 
@@ -158,13 +164,36 @@ hipError_t launch_mlp_adam(float* P, const float* g, float* m, float* v, int np,
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_forward(int lattice, const float* P, const float* X, int n,
+template <int NIN>
+using OpOf = std::conditional_t<NIN == 9, MlpShape<Lattice<LatticeType::D2Q9>>::Op,
+                                MlpShape<Lattice<LatticeType::D3Q27>>::Op>;
+
+size_t mlp_packed_bytes(int lattice) {
+  size_t b = 0;
+  by_lattice(lattice, [&](auto nin, auto h1, auto h2) {
+    b = MfmaLayout<decltype(nin)::value, decltype(h1)::value, decltype(h2)::value>::X;
+    return hipSuccess;
+  });
+  return b;
+}
+
+hipError_t launch_mlp_pack(int lattice, const float* P, void* packed, hipStream_t st) {
+  return by_lattice(lattice, [&](auto nin, auto h1, auto h2) {
+    constexpr int I = decltype(nin)::value;
+    hipLaunchKernelGGL((k_mlp_pack<I, decltype(h1)::value, decltype(h2)::value, OpOf<I>>),
+                       dim3(1), dim3(256), 0, st, P, static_cast<char*>(packed));
+    return hipGetLastError();
+  });
+}
+
+hipError_t launch_mlp_forward(int lattice, const void* packed, const float* X, int n,
                               double* alpha, hipStream_t st) {
   return by_lattice(lattice, [&](auto nin, auto h1, auto h2) {
     using M = MfmaLayout<decltype(nin)::value, decltype(h1)::value, decltype(h2)::value>;
+    using T = OpOf<decltype(nin)::value>;
     hipLaunchKernelGGL((k_mlp_forward<decltype(nin)::value, decltype(h1)::value,
-                                      decltype(h2)::value>),
-                       dim3((n + 255) / 256), dim3(256), M::BYTES, st, P, X, n, alpha);
+                                      decltype(h2)::value, T>),
+                       dim3((n + 255) / 256), dim3(256), M::BYTES, st, packed, X, n, alpha);
     return hipGetLastError();
   });
 }

@@ -21,8 +21,14 @@ hipError_t launch_mlp_sum(const float* G, int B, int np, float* g, hipStream_t s
 hipError_t launch_mlp_adam(float* P, const float* g, float* m, float* v, int np,
                            float lr, float b1, float b2, float eps, float c1,
                            float c2, hipStream_t st);
-// alpha_hat for raw feature rows X[n][nin].
-hipError_t launch_mlp_forward(int lattice, const float* P, const float* X, int n,
+// Bytes of the packed network (the inference kernels' LDS image of the
+// parameter block, weights in the matrix-core operand type).
+size_t mlp_packed_bytes(int lattice);
+// Packs the FP32 parameter block P into `packed` (mlp_packed_bytes bytes).
+hipError_t launch_mlp_pack(int lattice, const float* P, void* packed, hipStream_t st);
+// alpha_hat for raw feature rows X[n][nin] with the step kernel's inference
+// code, from a packed network.
+hipError_t launch_mlp_forward(int lattice, const void* packed, const float* X, int n,
                               double* alpha, hipStream_t st);
 
 }  // namespace lbm

@@ -52,7 +52,7 @@ struct StepArgs {
   // 1 and 2 with the residual acceptance (|G| <= residual_tol or certificate).
   int alpha_mode;
   double residual_tol;
-  const float* mlp;     // MLP parameter block (mode 1)
+  const void* mlp_packed;  // network in its LDS layout (alpha mode 1)
   // (features, alpha) sample log of Newton-solved cells (log_cap 0: off)
   float* log_x;
   double* log_alpha;

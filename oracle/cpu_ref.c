@@ -578,6 +578,198 @@ void ref_init_taylor_green(const ref_config *cfg, double u0, double *u) {
   free(t);
 }
 
+static int shell_of(int k2) {
+  int s = (int)sqrt((double)k2);
+  while ((2 * s + 1) * (2 * s + 1) <= 4 * k2) ++s;
+  while (s > 0 && (2 * s - 1) * (2 * s - 1) > 4 * k2) --s;
+  return s;
+}
+
+static void cross3(const double *a, const double *b, double *r) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+int ref_init_spectral_velocity(const ref_config *cfg, double urms, int kmax,
+                               double kpeak, uint64_t seed, double *u) {
+  const int D = ref_lattice_d(cfg->lattice);
+  const int n[3] = {cfg->nx, cfg->ny, cfg->nz};
+  if (kmax < 1) return -1;
+  for (int d = 0; d < D; ++d)
+    if (2 * kmax >= n[d]) return -1;
+  const int K = kmax, M = 2 * K + 1, Kz = D == 3 ? K : 0, Mz = 2 * Kz + 1;
+  const size_t nm = (size_t)M * M * Mz;
+  /* shell counts and amplitudes */
+  int *cnt = (int *)calloc((size_t)K + 1, sizeof(int));
+  for (int ix = 0; ix < M; ++ix)
+    for (int iy = 0; iy < M; ++iy)
+      for (int iz = 0; iz < Mz; ++iz) {
+        const int kx = ix - K, ky = iy - K, kz = iz - Kz;
+        const int s = shell_of(kx * kx + ky * ky + kz * kz);
+        if (s >= 1 && s <= K) cnt[s]++;
+      }
+  double *amp = (double *)calloc((size_t)K + 1, sizeof(double));
+  double esum = 0.0;
+  for (int s = 1; s <= K; ++s) {
+    const double r = s / kpeak;
+    const double e = (double)s * s * s * s * exp(-2.0 * r * r);
+    esum += e;
+    amp[s] = cnt[s] > 0 ? sqrt(e / cnt[s]) : 0.0;
+  }
+  /* draws: angles of the active half-space modes, in order */
+  size_t nh = 0;
+  for (int ix = 0; ix < M; ++ix)
+    for (int iy = 0; iy < M; ++iy)
+      for (int iz = 0; iz < Mz; ++iz) {
+        const int kx = ix - K, ky = iy - K, kz = iz - Kz;
+        const int s = shell_of(kx * kx + ky * ky + kz * kz);
+        const int half = kx > 0 || (kx == 0 && (ky > 0 || (ky == 0 && kz > 0)));
+        if (s >= 1 && s <= K && half) nh++;
+      }
+  double *ang = (double *)malloc(sizeof(double) * 3 * (nh ? nh : 1));
+  double *cs = (double *)malloc(sizeof(double) * 3 * (nh ? nh : 1));
+  double *sn = (double *)malloc(sizeof(double) * 3 * (nh ? nh : 1));
+  for (size_t j = 0; j < 3 * nh; ++j) {
+    const double unit = (double)(ref_splitmix64(seed, (uint64_t)j) >> 11) * 0x1.0p-53;
+    ang[j] = 2.0 * M_PI * unit;
+  }
+  /* cos and sin in separate loops (see trig_table) */
+  for (size_t j = 0; j < 3 * nh; ++j) cs[j] = cos(ang[j]);
+  for (size_t j = 0; j < 3 * nh; ++j) sn[j] = sin(ang[j]);
+  /* mode cube uh[mode][d] (re, im) */
+  double *ur = (double *)calloc(nm * D, sizeof(double));
+  double *ui = (double *)calloc(nm * D, sizeof(double));
+  size_t m = 0;
+  for (int ix = 0; ix < M; ++ix)
+    for (int iy = 0; iy < M; ++iy)
+      for (int iz = 0; iz < Mz; ++iz) {
+        const int kx = ix - K, ky = iy - K, kz = iz - Kz;
+        const int s = shell_of(kx * kx + ky * ky + kz * kz);
+        const int half = kx > 0 || (kx == 0 && (ky > 0 || (ky == 0 && kz > 0)));
+        if (!(s >= 1 && s <= K && half)) continue;
+        const double c1 = cs[3 * m], s1 = sn[3 * m];
+        const double c2 = cs[3 * m + 1], s2 = sn[3 * m + 1];
+        const double cp = cs[3 * m + 2], sp = sn[3 * m + 2];
+        ++m;
+        double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+        if (D == 3) {
+          const double k[3] = {(double)kx, (double)ky, (double)kz};
+          int a = 0;
+          if (fabs(k[1]) < fabs(k[a])) a = 1;
+          if (fabs(k[2]) < fabs(k[a])) a = 2;
+          double ea[3] = {0, 0, 0}, e1[3], e2[3], w[3];
+          ea[a] = 1.0;
+          cross3(k, ea, w);
+          double nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+          for (int d = 0; d < 3; ++d) e1[d] = w[d] / nw;
+          cross3(k, e1, w);
+          nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+          for (int d = 0; d < 3; ++d) e2[d] = w[d] / nw;
+          for (int d = 0; d < 3; ++d) {
+            re[d] = amp[s] * (cp * c1 * e1[d] + sp * c2 * e2[d]);
+            im[d] = amp[s] * (cp * s1 * e1[d] + sp * s2 * e2[d]);
+          }
+        } else {
+          const double nk = sqrt((double)(kx * kx + ky * ky));
+          const double e1[2] = {-ky / nk, kx / nk};
+          for (int d = 0; d < 2; ++d) {
+            re[d] = amp[s] * (c1 * e1[d]);
+            im[d] = amp[s] * (s1 * e1[d]);
+          }
+        }
+        const size_t p = ((size_t)ix * M + iy) * Mz + iz;
+        const size_t q = ((size_t)(M - 1 - ix) * M + (M - 1 - iy)) * Mz + (Mz - 1 - iz);
+        for (int d = 0; d < D; ++d) {
+          ur[p * D + d] = re[d];
+          ui[p * D + d] = im[d];
+          ur[q * D + d] = re[d];
+          ui[q * D + d] = -im[d];
+        }
+      }
+  const double gamma = urms * sqrt((double)D / esum);
+  /* twiddle tables */
+  double *tc[3], *ts[3];
+  for (int d = 0; d < 3; ++d) {
+    tc[d] = (double *)malloc(sizeof(double) * n[d]);
+    ts[d] = (double *)malloc(sizeof(double) * n[d]);
+    trig_table(n[d], ts[d], tc[d]);
+  }
+  const int nx = n[0], ny = n[1], nz = n[2];
+  /* pass z: t1[ix][iy][z][d] */
+  double *t1r = (double *)malloc(sizeof(double) * (size_t)M * M * nz * D);
+  double *t1i = (double *)malloc(sizeof(double) * (size_t)M * M * nz * D);
+#pragma omp parallel for schedule(static) num_threads(ref_threads)
+  for (int ixy = 0; ixy < M * M; ++ixy)
+    for (int z = 0; z < nz; ++z)
+      for (int d = 0; d < D; ++d) {
+        double re = 0.0, im = 0.0;
+        for (int iz = 0; iz < Mz; ++iz) {
+          const int kz = iz - Kz;
+          const int t = (int)((((int64_t)kz * z) % nz + nz) % nz);
+          const double c = tc[2][t], sv = ts[2][t];
+          const size_t p = ((size_t)ixy * Mz + iz) * D + d;
+          const double a = ur[p], b = ui[p];
+          const double tr = a * c - b * sv;
+          const double ti = a * sv + b * c;
+          re += tr;
+          im += ti;
+        }
+        const size_t o = ((size_t)ixy * nz + z) * D + d;
+        t1r[o] = re;
+        t1i[o] = im;
+      }
+  /* pass y: t2[ix][y][z][d] */
+  double *t2r = (double *)malloc(sizeof(double) * (size_t)M * ny * nz * D);
+  double *t2i = (double *)malloc(sizeof(double) * (size_t)M * ny * nz * D);
+#pragma omp parallel for schedule(static) num_threads(ref_threads)
+  for (int ix = 0; ix < M; ++ix)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z)
+        for (int d = 0; d < D; ++d) {
+          double re = 0.0, im = 0.0;
+          for (int iy = 0; iy < M; ++iy) {
+            const int ky = iy - K;
+            const int t = (int)((((int64_t)ky * y) % ny + ny) % ny);
+            const double c = tc[1][t], sv = ts[1][t];
+            const size_t p = (((size_t)ix * M + iy) * nz + z) * D + d;
+            const double a = t1r[p], b = t1i[p];
+            const double tr = a * c - b * sv;
+            const double ti = a * sv + b * c;
+            re += tr;
+            im += ti;
+          }
+          const size_t o = (((size_t)ix * ny + y) * nz + z) * D + d;
+          t2r[o] = re;
+          t2i[o] = im;
+        }
+  /* pass x: u[d][cell] = gamma Re(...) */
+  const int64_t N = ref_cells(cfg);
+#pragma omp parallel for schedule(static) num_threads(ref_threads)
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z)
+        for (int d = 0; d < D; ++d) {
+          double re = 0.0;
+          for (int ix = 0; ix < M; ++ix) {
+            const int kx = ix - K;
+            const int t = (int)((((int64_t)kx * x) % nx + nx) % nx);
+            const double c = tc[0][t], sv = ts[0][t];
+            const size_t p = (((size_t)ix * ny + y) * nz + z) * D + d;
+            const double tr = t2r[p] * c - t2i[p] * sv;
+            re += tr;
+          }
+          u[(int64_t)d * N + ((int64_t)x * ny + y) * nz + z] = gamma * re;
+        }
+  for (int d = 0; d < 3; ++d) {
+    free(tc[d]);
+    free(ts[d]);
+  }
+  free(t1r); free(t1i); free(t2r); free(t2i);
+  free(ur); free(ui); free(ang); free(cs); free(sn); free(amp); free(cnt);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Diagnostics.                                                              */
 

@@ -140,6 +140,32 @@ void ref_init_noise_velocity(const ref_config *cfg, uint64_t seed, double amp,
  * u = U0 (sin x cos y, -cos x sin y) in 2D. */
 void ref_init_taylor_green(const ref_config *cfg, double u0, double *u);
 
+/* Solenoidal random-phase velocity field with shell spectrum
+ * E(s) = s^4 exp(-2 (s / kpeak)^2) on shells s = 1..kmax (no reference
+ * counterpart: the reference declares InitVelocityType::Decay without an
+ * implementation, Options.h:27).  Specification:
+ *  - modes: integer k with |k_d| <= kmax (k_z = 0 in 2D); shell s(k) = the
+ *    integer nearest |k|, i.e. (2s-1)^2 <= 4|k|^2 < (2s+1)^2; a mode is
+ *    active when 1 <= s <= kmax; N_s = number of active modes of shell s;
+ *  - half space H: k_x > 0, or k_x = 0 and k_y > 0, or k_x = k_y = 0 and
+ *    k_z > 0.  Active modes of H in ascending (k_x, k_y, k_z) order, m-th of
+ *    them, take draws j = 3m, 3m+1, 3m+2 of ref_splitmix64(seed, j) as
+ *    U = (r >> 11) 2^-53 and set theta1 = 2 pi U_0, theta2 = 2 pi U_1,
+ *    phi = 2 pi U_2;
+ *  - basis: 3D: a = axis of the smallest |k_d| (ties: lowest d),
+ *    e1 = k x a / |k x a|, e2 = k x e1 / |k x e1|; 2D: e1 = (-k_y, k_x)/|k|;
+ *  - amplitude: A_s = sqrt(E(s) / N_s); 3D: uh(k) = A_s (cos phi e^{i theta1} e1
+ *    + sin phi e^{i theta2} e2); 2D: uh(k) = A_s e^{i theta1} e1;
+ *    uh(-k) = conj(uh(k));
+ *  - field: u(x) = gamma Re sum_k uh(k) exp(2 pi i (k_x x/nx + k_y y/ny + k_z z/nz)),
+ *    gamma = urms sqrt(D / sum_s E(s)), so that mean |u|^2 = D urms^2 exactly
+ *    (Parseval; requires 2 kmax < each extent).  Evaluated as three separable
+ *    passes (sum over k_z, then k_y, then k_x, each in ascending order) with
+ *    twiddles from tables of cos/sin(2 pi m / n).
+ * Returns 0, or -1 when kmax < 1 or 2 kmax >= an extent. */
+int ref_init_spectral_velocity(const ref_config *cfg, double urms, int kmax,
+                               double kpeak, uint64_t seed, double *u);
+
 /* Diagnostics on stored fields, FP64 with a fixed pairwise summation tree:
  *   mass = sum rho;  energy = (1/N) sum 1/2 rho |u|^2 (Analysis.h:53-61);
  *   enstrophy = (1/N) sum 1/2 |w|^2 with w the curl of u by second-order

@@ -84,6 +84,9 @@ def lib():
         L.ref_splitmix64.restype = ctypes.c_uint64
         L.ref_init_noise_velocity.argtypes = [cp, ctypes.c_uint64, ctypes.c_double, dp]
         L.ref_init_taylor_green.argtypes = [cp, ctypes.c_double, dp]
+        L.ref_init_spectral_velocity.argtypes = [cp, ctypes.c_double, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_uint64, dp]
+        L.ref_init_spectral_velocity.restype = ctypes.c_int
         L.ref_reduce.argtypes = [cp, dp, dp, dp, dp, dp]
         L.ref_set_threads.argtypes = [ctypes.c_int]
         L.mlp_ref_param_count.argtypes = [ctypes.c_int] * 3
@@ -195,6 +198,18 @@ def taylor_green(cfg: Config, u0: float) -> np.ndarray:
     u = np.zeros((cfg.d, cfg.cells))
     k = cfg.c()
     lib().ref_init_taylor_green(ctypes.byref(k), u0, _p(u))
+    return u
+
+
+def spectral_velocity(cfg: Config, urms: float, kmax: int, kpeak: float,
+                      seed: int) -> np.ndarray:
+    """Solenoidal random-phase field with E(s) = s^4 exp(-2 (s/kpeak)^2) on
+    shells 1..kmax and per-component rms `urms` (cpu_ref.h states the spec)."""
+    u = np.zeros((cfg.d, cfg.cells))
+    k = cfg.c()
+    rc = lib().ref_init_spectral_velocity(ctypes.byref(k), urms, kmax, kpeak, seed, _p(u))
+    if rc != 0:
+        raise ValueError("spectral init needs 1 <= kmax and 2 kmax < every extent")
     return u
 
 

@@ -55,12 +55,13 @@ def test_mlp_gradient_and_forward_match_oracle(oracle, lat):
     err = np.abs(g.astype(np.float64) - gref)
     assert np.all(err <= 1e-4 * np.abs(gref).max() + 1e-4 * np.abs(gref)), err.max()
     yref = oracle.mlp_forward(p.astype(np.float64), nin, h1, h2, z)
-    # Inference runs on the matrix cores with f16 operands (features, weights,
-    # hidden activations) and FP32 accumulation; biases stay FP32.  Forward
-    # rounding-error bound with u = 2^-11 per f16 rounding, layer by layer:
+    # Inference runs on the matrix cores with f16 (D2Q9) or bf16 (D3Q27)
+    # operands (features, weights, hidden activations) and FP32 accumulation;
+    # biases stay FP32.  Forward rounding-error bound with unit roundoff u per
+    # operand rounding (f16: 2^-11, bf16: 2^-8), layer by layer:
     #   e1 = 2u |W1||z| + u |a1|,  e2 = |W2| e1 + 2u |W2||a1| + u |a2|,
     #   e3 = |W3| e2 + 2u |W3||a2|   (ReLU is 1-Lipschitz), doubled for slack.
-    u = 2.0 ** -11
+    u = 2.0 ** -11 if lat == LatticeType.D2Q9 else 2.0 ** -8
     P = p.astype(np.float64)
     W1 = np.abs(P[:h1 * nin].reshape(h1, nin))
     o = h1 * nin + h1
@@ -76,7 +77,7 @@ def test_mlp_gradient_and_forward_match_oracle(oracle, lat):
     out_scale = np.float64(np.float32(1e-3))
     bound = 2.0 * out_scale * e3 + 1e-12
     err = np.abs(pred - (2.0 + out_scale * yref))
-    say(f"f16 inference |err| max {err.max():.3e} median {np.median(err):.3e}; "
+    say(f"matrix-core inference |err| max {err.max():.3e} median {np.median(err):.3e}; "
         f"bound median {np.median(bound):.3e}; output range {out_scale * np.ptp(yref):.3e}")
     assert np.all(err <= bound), (err / bound).max()
 

@@ -116,6 +116,41 @@ def test_init_kernels_match_oracle(oracle, lat, eq):
     np.testing.assert_array_equal(fe, oracle.init_equilibrium(oc, rho, u))
 
 
+SPECTRAL = [(D3Q27, (24, 20, 18), 6, 3.0), (D3Q19, (18, 16, 20), 5, 2.0),
+            (D2Q9, (22, 26, 1), 5, 2.5)]
+
+
+@pytest.mark.parametrize("lat,shape,kmax,kpeak", SPECTRAL,
+                         ids=[f"{c[0].name}" for c in SPECTRAL])
+def test_spectral_init_and_entropic_steps_match_oracle(oracle, lat, shape, kmax, kpeak):
+    """The spectral initialiser on the GPU against the oracle's, then 10
+    FP64 entropic steps from that state (a small decaying-turbulence case)."""
+    nx, ny, nz = shape
+    cfg = Config(lattice=lat, collision=CollisionType.ELBM, nx=nx, ny=ny, nz=nz, tau=0.50006)
+    oc = oracle_cfg(oracle, cfg)
+    say(f"case spectral {lat.name} {shape} kmax={kmax}")
+    with Algorithm(cfg) as a:
+        a.init_spectral(1.0, 0.03, kmax, kpeak, 42)
+        f0 = a.populations()
+        a.iterate(10, store=True)
+        f = a.populations()
+        rho, uh, al, _ = a.store_fields()
+        st = a.elbm_stats()
+        with pytest.raises(Exception):
+            a.init_spectral(1.0, 0.03, min(nx, ny) // 2, kpeak, 42)
+    u0 = oracle.spectral_velocity(oc, 0.03, kmax, kpeak, 42)
+    fo = oracle.init_equilibrium(oc, np.ones(oc.cells), u0)
+    say(f"init rel err {rel_err(f0, fo):.3e} bitwise={np.array_equal(f0, fo)}")
+    assert rel_err(f0, fo) <= 1e-14
+    ref = oracle.run(oc, fo, 10)
+    say(f"oracle branches {ref.branch.tolist()}; gpu {st}")
+    assert ref.branch[2] > 0, "the case must exercise Newton"
+    assert rel_err(f, ref.f) <= 1e-12
+    assert rel_err(rho, ref.rho) <= 1e-12
+    assert rel_err(uh, ref.u) <= 1e-12
+    np.testing.assert_allclose(al, ref.alpha, rtol=1e-9)
+
+
 @pytest.mark.parametrize("lat", [D2Q9, D3Q19, D3Q27])
 def test_elbm_fp64_parity(oracle, lat):
     nx, ny, nz = SHAPES[lat]

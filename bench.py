@@ -314,13 +314,83 @@ def accuracy_study(nsteps: int, every: int, n: int) -> dict:
     return out
 
 
+def decay_study(nsteps: int, every: int, n: int) -> dict:
+    """Energy and enstrophy of D3Q27 entropic decaying turbulence (n^3,
+    tau = 0.50006, spectral initial field on shells 1..16 peaked at 4 with
+    per-component rms 0.1 cs, seed 42) with FP32 compute and shifted FP32
+    storage -- reference Newton, residual Newton from the stored alpha, learned
+    alpha -- against the FP64 reference-order build (equal to the FP64 oracle
+    to <= 1e-12 on every tested configuration).  E and Omega every `every` steps
+    (E = (1/N) sum 1/2 rho |u_h|^2, Omega = (1/N) sum 1/2 |curl u_h|^2 by
+    central differences).  The learned run logs (features, alpha) over its
+    first `every` steps with the Newton solver, trains, then switches."""
+    import numpy as np
+
+    import metalbm_amd as m
+    base = dict(lattice=m.LatticeType.D3Q27, collision=m.CollisionType.ELBM, nx=n, ny=n, nz=n,
+                tau=0.50006, forcing_scheme=m.ForcingSchemeType.None_, force=m.ForceType.None_,
+                residual_tol=1e-10)
+    f32 = dict(storage=m.Precision.F32, compute=m.Precision.F32, shifted=True)
+    runs = {"f64-reference": (dict(storage=m.Precision.F64, compute=m.Precision.F64,
+                                   shifted=False), m.AlphaMode.NEWTON),
+            "f32-newton": (f32, m.AlphaMode.NEWTON),
+            "f32-warm-residual": (f32, m.AlphaMode.WARM_RESIDUAL),
+            "f32-learned": (f32, m.AlphaMode.LEARNED)}
+    urms = 0.1 / 3 ** 0.5
+    series, stats = {}, {}
+    for name, (prec, mode) in runs.items():
+        t0 = time.perf_counter()
+        with m.Algorithm(m.Config(**base, **prec)) as a:
+            a.init_spectral(1.0, urms, 16, 4.0, 42)
+            out = []
+            for k in range(nsteps // every):
+                if mode == m.AlphaMode.LEARNED and k == 0:
+                    a.sample_log_enable(500000, 1e-3, 1)
+                    a.iterate(every, store=True)
+                    a.sample_log_disable()
+                    a.mlp_train(3000, 4096, 2e-3, seed=1)
+                    a.set_alpha_mode(mode, 1e-10)
+                    a.reset_elbm_stats()
+                else:
+                    if k == 0:
+                        a.set_alpha_mode(mode, 1e-10)
+                    a.iterate(every, store=True)
+                out.append(a.reduce()[1:])
+                if (k + 1) % 10 == 0:
+                    log(f"decay {name}: step {(k + 1) * every}, E, Omega = {out[-1]}")
+            stats[name] = a.elbm_stats()
+        series[name] = np.array(out)
+        log(f"decay {name}: E, Omega at {nsteps} = {out[-1]} ({time.perf_counter() - t0:.1f} s) "
+            f"{stats[name]}")
+    ref = series["f64-reference"]
+    res = {"workload": f"D3Q27 SRT Entropic {n}^3 decaying turbulence tau=0.50006, {nsteps} "
+                       f"steps, E and Omega every {every}",
+           "reference_E_first_last": [ref[0, 0], ref[-1, 0]],
+           "reference_Omega_first_last": [ref[0, 1], ref[-1, 1]]}
+    for name, v in series.items():
+        if name == "f64-reference":
+            res[name] = {"elbm_stats": stats[name]}
+            continue
+        rel = np.abs(v - ref) / np.abs(ref)
+        res[name] = {"max_rel_E_error": float(rel[:, 0].max()),
+                     "max_rel_Omega_error": float(rel[:, 1].max()),
+                     "final_rel_E_error": float(rel[-1, 0]),
+                     "final_rel_Omega_error": float(rel[-1, 1]), "elbm_stats": stats[name]}
+    return res
+
+
 def traffic_of(name):
-    p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    """HBM bytes per launch of the workload's step kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_<name>.txt,
+    counters scaled by a copy of known size), or None when not measured."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{name}.txt")
     if not os.path.exists(p):
         return None
     with open(p) as fh:
-        d = json.load(fh)
-    return d.get("hbm_bytes_per_launch")
+        for line in fh:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    return None
 
 
 def main(argv=None) -> int:
@@ -336,7 +406,15 @@ def main(argv=None) -> int:
                     help="D3Q27 Taylor-Green kinetic-energy decay, FP16/FP32/FP64 shifted "
                          "storage vs the FP64 build (prints its own JSON line)")
     ap.add_argument("--accuracy-n", type=int, default=384)
+    ap.add_argument("--decay", action="store_true",
+                    help="D3Q27 entropic decaying turbulence: energy and enstrophy of the FP32 "
+                         "solvers vs the FP64 build (prints its own JSON line)")
+    ap.add_argument("--decay-n", type=int, default=256)
     args = ap.parse_args(argv)
+    if args.decay:
+        res = decay_study(args.steps or 3000, 100, args.decay_n)
+        print(json.dumps({"metric": "energy/enstrophy error vs FP64", **res}))
+        return 0
     if args.accuracy:
         res = accuracy_study(args.steps or 2000, 100, args.accuracy_n)
         print(json.dumps({"metric": "kinetic-energy decay error vs FP64", **res}))

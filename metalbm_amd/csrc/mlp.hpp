@@ -363,12 +363,14 @@ __device__ __forceinline__ double residual_curvature(const V& p, const double* x
 // 31-45): accepts alpha once |G| <= tol, or after a Newton step s whose
 // Taylor remainder bound 1/2 s^2 max|G''| (G'' is convex in alpha, so its
 // maximum over the step lies at an end point) is <= tol, with a relative
-// rounding allowance.  Every evaluated point lies in (1, amax), where all
-// logarithms are defined: a Newton step that would leave the interval is
-// replaced by the midpoint towards the violated end (G is concave with
-// G(0) = 0, so the iteration then approaches the root from the right).
-// Returns 2 with branch 3 when 50 evaluations do not converge or the step is
-// not finite.  p views f and fneq in FP64.
+// rounding allowance.  G is concave with G(0) = 0, so on (0, inf) it is
+// positive below its non-trivial root and negative above it: every evaluation
+// narrows a bracket (lo, hi), initially (1, amax), around the root, and a
+// Newton step that leaves the bracket (e.g. towards the trivial root from
+// below the maximum of G) is replaced by its midpoint.  All evaluated points
+// lie in (1, amax), where every logarithm is defined.  Returns 2 with branch 3
+// when 50 evaluations do not converge (no root in (1, amax)) or a step is not
+// finite.  p views f and fneq in FP64.
 template <class L, class V>
 __device__ __forceinline__ double residual_newton(const V& p, double amax,
                                                   double start, double tol,
@@ -399,6 +401,7 @@ __device__ __forceinline__ double residual_newton(const V& p, double amax,
   else
     sums([](double y) { return log1p(y); });
   double a = (start > 1.0 && start < amax) ? start : 2.0;
+  double lo = 1.0, hi = amax;
   evals = 0;
   first = false;
   for (int it = 0; it < 50; ++it) {
@@ -410,11 +413,14 @@ __device__ __forceinline__ double residual_newton(const V& p, double amax,
       first = it == 0;
       return a;
     }
+    if (g > 0.0)
+      lo = a;
+    else
+      hi = a;
     const double s = g / dg;
-    if (!isfinite(s)) break;
     const double an = a - s;
-    if (!(an > 1.0 && an < amax)) {
-      a = an >= amax ? 0.5 * (a + amax) : 0.5 * (a + 1.0);
+    if (!(an > lo && an < hi)) {  // also catches a non-finite step
+      a = 0.5 * (lo + hi);
       continue;
     }
     const double bound =

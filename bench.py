@@ -65,9 +65,9 @@ WORKLOADS = {
                  wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True),
         init=("noise", 1e-3, 42), steps=200),
     "d2q9-elbm-2048-warm": dict(
-        desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, alpha by residual Newton "
-             "(|G| <= 1e-10 or Taylor certificate) from the stored alpha: the learned "
-             "solver without its network",
+        desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, the learned solver "
+             "without its network: the stored alpha, one Newton correction accepted by "
+             "|G| <= 1e-10 or its Taylor certificate, else the reference Newton",
         cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
                  forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
                  wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True,
@@ -77,7 +77,8 @@ WORKLOADS = {
         desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, alpha from the MLP "
              "(9 normalised non-equilibrium moments -> 32 -> 32 -> 1, FP32 weights) trained "
              "on the GPU on (features, alpha) logged from the first 1000 Newton steps, then "
-             "residual Newton (|G| <= 1e-10 or Taylor certificate), Newton fallback",
+             "one Newton correction accepted by |G| <= 1e-10 or its Taylor certificate, "
+             "else the reference Newton",
         cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
                  forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
                  wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True,
@@ -94,8 +95,9 @@ WORKLOADS = {
                  shifted=True),
         init=("spectral", 0.1 / 3 ** 0.5, 16, 4.0, 42), steps=50),
     "d3q27-elbm-256-warm": dict(
-        desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, alpha by residual Newton "
-             "(|G| <= 1e-10 or Taylor certificate) from the stored alpha",
+        desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, the learned solver "
+             "without its network (stored alpha, one certified correction, else the "
+             "reference Newton)",
         cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
                  forcing_scheme="None_", force="None_", storage="F32", compute="F32",
                  shifted=True, alpha_mode="WARM_RESIDUAL", residual_tol=1e-10),
@@ -104,7 +106,8 @@ WORKLOADS = {
         desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, alpha from the MLP "
              "(27 normalised non-equilibrium moments -> 64 -> 64 -> 1, BF16 weights on the "
              "matrix cores, FP32 accumulate) trained on the GPU on (features, alpha) logged "
-             "from the first 200 Newton steps, then residual Newton, Newton fallback",
+             "from the first 200 Newton steps, one certified correction, else the reference "
+             "Newton",
         cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
                  forcing_scheme="None_", force="None_", storage="F32", compute="F32",
                  shifted=True, alpha_mode="NEWTON", residual_tol=1e-10),
@@ -225,11 +228,11 @@ def init(a, spec):
         raise ValueError(kind)
 
 
-def bytes_per_cell(cfg, wl_cfg, learned: bool = False) -> int:
+def bytes_per_cell(cfg, wl_cfg) -> int:
     q = cfg.q
     b = 2 * q * SIZES[wl_cfg["storage"]]
-    # The learned solver keeps no alpha state; the Newton ones read and write it.
-    if wl_cfg.get("collision") == "ELBM" and not learned:
+    # Every entropic solver reads and writes the alpha state (Algorithm.h:103-106).
+    if wl_cfg.get("collision") == "ELBM":
         b += 2 * (8 if wl_cfg["compute"] == "F64" else 4)   # alpha warm-start state r+w
     return b
 
@@ -318,8 +321,8 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
     """Energy and enstrophy of D3Q27 entropic decaying turbulence (n^3,
     tau = 0.50006, spectral initial field on shells 1..16 peaked at 4 with
     per-component rms 0.1 cs, seed 42) with FP32 compute and shifted FP32
-    storage -- reference Newton, residual Newton from the stored alpha, learned
-    alpha -- against the FP64 reference-order build (equal to the FP64 oracle
+    storage -- reference Newton, certified correction of the stored alpha,
+    learned alpha -- against the FP64 reference-order build (equal to the FP64 oracle
     to <= 1e-12 on every tested configuration).  E and Omega every `every` steps
     (E = (1/N) sum 1/2 rho |u_h|^2, Omega = (1/N) sum 1/2 |curl u_h|^2 by
     central differences).  The learned run logs (features, alpha) over its
@@ -475,7 +478,7 @@ def main(argv=None) -> int:
     N = cfg.cells
     mlups = N * K / wall / 1e6
     kern_s = ev_ms / 1e3 / K
-    bpc = bytes_per_cell(cfg, wl["cfg"], learned=bool(learn))
+    bpc = bytes_per_cell(cfg, wl["cfg"])
     achieved = bpc * N / kern_s / 1e9
     log(f"wall {wall * 1e3:.2f} ms for {K} steps, events {ev_ms:.2f} ms, "
         f"{mlups:.1f} MLUPS, {achieved:.1f} GB/s algorithmic")

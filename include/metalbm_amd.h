@@ -74,13 +74,15 @@ enum {
   LBM_ALPHA_NEWTON = 0,   /* reference Newton: warm start from the stored
                              alpha, |step| <= 1e-8, <= 50 iterations
                              (Collision.h:328-375, EntropicStep.h:111-140) */
-  LBM_ALPHA_LEARNED = 1,  /* start from the MLP's alpha, then residual Newton:
-                             accept when |G| <= residual_tol or when the
-                             Taylor bound of the next Newton point is;
-                             alpha = 2 fallback as the reference.  No alpha
-                             state is read or written. */
-  LBM_ALPHA_WARM_RESIDUAL = 2  /* the same residual Newton started from the
-                             stored alpha: the method without the network */
+  LBM_ALPHA_LEARNED = 1,  /* start from the MLP's alpha and make one Newton
+                             correction: accept the start when |G| <=
+                             residual_tol, or the corrected point when its
+                             Taylor remainder bound is <= residual_tol; else
+                             the reference Newton from the stored alpha
+                             decides (with its alpha = 2 fallback).  The
+                             alpha state is kept as for NEWTON. */
+  LBM_ALPHA_WARM_RESIDUAL = 2  /* the same with the stored alpha as the start:
+                             the method without the network */
 };
 
 typedef struct {
@@ -97,7 +99,8 @@ typedef struct {
   int shifted;          /* nonzero: store f_i - w_i instead of f_i */
   int compute;          /* LBM_F64 | LBM_F32 */
   int alpha_mode;       /* LBM_ALPHA_* (ELBM) */
-  double residual_tol;  /* H-residual tolerance of LBM_ALPHA_LEARNED */
+  double residual_tol;  /* absolute H-residual tolerance |G(alpha)| of the
+                           LEARNED and WARM_RESIDUAL solvers */
   int kernel;           /* LBM_KERNEL_* */
 } lbm_config;
 

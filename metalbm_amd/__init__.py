@@ -78,8 +78,8 @@ class KernelShape(enum.IntEnum):
 
 class AlphaMode(enum.IntEnum):
     NEWTON = 0          # reference Newton (Collision.h:328-375)
-    LEARNED = 1         # MLP start + residual Newton with Taylor certificate
-    WARM_RESIDUAL = 2   # the same residual Newton from the stored alpha
+    LEARNED = 1         # MLP start + one certified Newton correction, else NEWTON
+    WARM_RESIDUAL = 2   # the same from the stored alpha (no network)
 
 
 _QD = {LatticeType.D2Q9: (9, 2), LatticeType.D3Q19: (19, 3), LatticeType.D3Q27: (27, 3)}

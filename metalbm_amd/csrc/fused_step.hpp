@@ -232,8 +232,9 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 }
 
 // Entropic step, one instantiation per alpha solver MODE (0: reference
-// Newton, 1: learned start + residual Newton, 2: residual Newton from the
-// stored alpha) so that no kernel carries the registers of another's solver.
+// Newton; 1: network start, 2: stored-alpha start, each with one certified
+// correction and the reference Newton as fallback) so that no kernel carries
+// the registers of another's solver.
 // Three phases so that the learned mode's wavefront-wide matrix-core
 // inference runs with every lane present: (1) pull, moments, fneq and the
 // alpha prelude per cell; (2) MODE 1 only: the network predicts alpha for
@@ -258,9 +259,9 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
   C f[Q], fneq[Q], eqv[Q], F[3];
   C rho = C(1), u[3] = {C(0), C(0), C(0)}, u2 = C(0), drho = C(0);
   double alpha = 2.0, amax = 2.5;
-  // The learned mode needs no warm start: its alpha state is neither read
-  // nor written (Algorithm.h:103-106 does both for the reference).
-  constexpr bool warm = MODE != 1;
+  // Every mode keeps the alpha state (Algorithm.h:103-106): it is the
+  // reference Newton's warm start, also when that Newton is the fallback of
+  // the residual modes.
   // ---- phase 1 ----
   if (c.active) {
     cell = self_index<L>(a, c);
@@ -314,11 +315,23 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
     C* __restrict__ astate = static_cast<C*>(a.alpha_state);
     if (solve) {
       const Fp64View<L, C, SH> p{f, fneq};
-      if constexpr (warm) start = double(astate[cell]);
-      if constexpr (MODE == 0)
-        alpha = newton_reference<L>(p, start, amax, branch, its);
-      else
-        alpha = residual_newton<L>(p, amax, start, a.residual_tol, branch, its, first);
+      if constexpr (MODE == 0) {
+        alpha = newton_reference<L>(p, double(astate[cell]), amax, branch, its);
+      } else {
+        // MODE 1 starts from the network's alpha, MODE 2 from the stored one;
+        // one certified correction, else the reference Newton decides.
+        if constexpr (MODE == 2) start = double(astate[cell]);
+        int ev = 0;
+        if (residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev)) {
+          branch = 2;
+          first = true;
+          its = ev;
+        } else {
+          int nit = 0;
+          alpha = newton_reference<L>(p, double(astate[cell]), amax, branch, nit);
+          its = ev + nit;
+        }
+      }
       if constexpr (Shape::NIN > 0) {
         if (a.log_cap > 0 && branch == 2 &&
             sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {
@@ -330,7 +343,7 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
         }
       }
     }
-    if constexpr (warm) astate[cell] = C(alpha);  // Algorithm.h:106
+    astate[cell] = C(alpha);  // Algorithm.h:106
     const C gc = C(a.guo_coef);
     // tau = 1 / (alpha beta); collide with 1 / tau (Collision.h:240, 254).
     const double tau = 1.0 / (alpha * a.beta);

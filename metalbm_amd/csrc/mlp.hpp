@@ -3,7 +3,8 @@
 // No reference counterpart: the reference finds alpha by Newton only
 // (Collision.h:328-349).  Here a small ReLU MLP maps the cell's normalised
 // non-equilibrium moments to a first guess of alpha; the Newton loop below
-// (residual_newton) then certifies or corrects it.
+// (residual_correction) then certifies or corrects it, and the reference
+// Newton takes over when it cannot.
 //
 // Features: m_k = (1/rho) sum_i fneq_i prod_d phi_{a_kd}(c_id), phi_0 = 1,
 // phi_1 = c, phi_2 = c^2, over the full tensor basis a_k in {0,1,2}^D (the
@@ -237,7 +238,7 @@ __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
   return double(fl[H1 + H2 + 1 + 2 * NIN]) + double(fl[H1 + H2 + 2 + 2 * NIN]) * double(y);
 }
 
-// ---- residual Newton with a Taylor certificate ------------------------------
+// ---- certified Newton correction ------------------------------------------
 //
 // G(a) = H(f) - H(f - a fneq) = sum_i f_i [a x_i A_i - (1 - a x_i) log1p(-a x_i)]
 // with x_i = fneq_i / f_i and A_i = ln(f_i / w_i): the reference's H-difference
@@ -245,10 +246,8 @@ __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
 // as in EntropicStep.h:48-62, and G''(a) = -sum f x^2 / (1 - a x) < 0 with |G''|
 // convex in a.  After a Newton step s = G/G' from a, Taylor's theorem gives
 // |G(a - s)| <= s^2/2 max_[a, a - s] |G''| = s^2/2 max(|G''(a)|, |G''(a - s)|),
-// a certificate that needs no logarithm.  A cell is accepted when |G| <= tol at
-// an evaluated point, or when the certificate of the next point is <= tol.
-// Outcome codes: 2 Newton accepted, 3 fallback (alpha = 2); `first` reports
-// acceptance from the starting point's single evaluation (plus certificate).
+// a certificate that needs no logarithm.  A start is accepted when |G| <= tol
+// there, or its Newton point when the certificate is <= tol.
 
 // log1p(y) for |y| <= kSeriesMax by its Taylor series to degree 12
 // (truncation < |y|^13 / 13 < 1e-20 |y|): 12 FMAs instead of a library call.
@@ -359,24 +358,21 @@ __device__ __forceinline__ double residual_curvature(const V& p, const double* x
   return s;
 }
 
-// Residual Newton for G(alpha) = 0 (G in the log1p form of EntropicStep.h:
-// 31-45): accepts alpha once |G| <= tol, or after a Newton step s whose
-// Taylor remainder bound 1/2 s^2 max|G''| (G'' is convex in alpha, so its
-// maximum over the step lies at an end point) is <= tol, with a relative
-// rounding allowance.  G is concave with G(0) = 0, so on (0, inf) it is
-// positive below its non-trivial root and negative above it: every evaluation
-// narrows a bracket (lo, hi), initially (1, amax), around the root, and a
-// Newton step that leaves the bracket (e.g. towards the trivial root from
-// below the maximum of G) is replaced by its midpoint.  All evaluated points
-// lie in (1, amax), where every logarithm is defined.  Returns 2 with branch 3
-// when 50 evaluations do not converge (no root in (1, amax)) or a step is not
-// finite.  p views f and fneq in FP64.
+// One residual-checked Newton correction of a starting alpha (G in the log1p
+// form of EntropicStep.h:31-45).  Accepts the start if |G(start)| <= tol,
+// else the Newton point a1 = start - G/G' if it lies in (1, amax) and its
+// Taylor certificate 1/2 s^2 max(|G''(start)|, |G''(a1)|) (G'' is convex in
+// alpha, so its maximum over the step lies at an end point) is <= tol, with a
+// relative rounding allowance.  Returns false -- the caller then runs the
+// reference Newton -- when the start is outside (1, amax) or neither test
+// passes.  evals: G evaluations made (0 or 1).  p views f and fneq in FP64.
 template <class L, class V>
-__device__ __forceinline__ double residual_newton(const V& p, double amax,
-                                                  double start, double tol,
-                                                  int& branch, int& evals,
-                                                  bool& first) {
+__device__ __forceinline__ bool residual_correction(const V& p, double amax,
+                                                    double start, double tol,
+                                                    double& alpha, int& evals) {
   constexpr int Q = L::dimQ;
+  evals = 0;
+  if (!(start > 1.0 && start < amax)) return false;
   double x[Q];
   double ymax = 0.0;
   static_for<Q>([&](auto ic) {
@@ -400,40 +396,21 @@ __device__ __forceinline__ double residual_newton(const V& p, double amax,
     sums([](double y) { return log1p_mid(y); });
   else
     sums([](double y) { return log1p(y); });
-  double a = (start > 1.0 && start < amax) ? start : 2.0;
-  double lo = 1.0, hi = amax;
-  evals = 0;
-  first = false;
-  for (int it = 0; it < 50; ++it) {
-    double g, dg, h;
-    residual_terms<L>(p, x, S, Sn, a, g, dg, h);
-    ++evals;
-    if (fabs(g) <= tol) {
-      branch = 2;
-      first = it == 0;
-      return a;
-    }
-    if (g > 0.0)
-      lo = a;
-    else
-      hi = a;
-    const double s = g / dg;
-    const double an = a - s;
-    if (!(an > lo && an < hi)) {  // also catches a non-finite step
-      a = 0.5 * (lo + hi);
-      continue;
-    }
-    const double bound =
-        0.5 * s * s * fmax(h, residual_curvature<L>(p, x, an)) * (1.0 + 1e-6);
-    if (bound <= tol) {
-      branch = 2;
-      first = it == 0;
-      return an;
-    }
-    a = an;
+  double g, dg, h;
+  residual_terms<L>(p, x, S, Sn, start, g, dg, h);
+  evals = 1;
+  if (fabs(g) <= tol) {
+    alpha = start;
+    return true;
   }
-  branch = 3;
-  return 2.0;
+  const double s = g / dg;
+  const double a1 = start - s;
+  if (!(a1 > 1.0 && a1 < amax)) return false;  // also a non-finite step
+  const double bound =
+      0.5 * s * s * fmax(h, residual_curvature<L>(p, x, a1)) * (1.0 + 1e-6);
+  if (!(bound <= tol)) return false;
+  alpha = a1;
+  return true;
 }
 
 }  // namespace lbm

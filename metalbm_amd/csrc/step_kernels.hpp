@@ -48,8 +48,9 @@ struct StepArgs {
   double* F_out;        // [D][N]
   void* alpha_state;    // ELBM warm-start field (compute type)
   unsigned long long* stats;  // 6 ELBM counters or NULL
-  // Entropic alpha solver: 0 reference Newton, 1 learned start, 2 warm start,
-  // 1 and 2 with the residual acceptance (|G| <= residual_tol or certificate).
+  // Entropic alpha solver: 0 reference Newton; 1 network start, 2 stored
+  // alpha start, each with one certified correction (|G| <= residual_tol or
+  // Taylor bound), else the reference Newton.
   int alpha_mode;
   double residual_tol;
   const void* mlp_packed;  // network in its LDS layout (alpha mode 1)

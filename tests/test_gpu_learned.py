@@ -4,12 +4,14 @@
   the FP64 reference network (oracle/mlp_ref.c) on the same weights.
 * End to end: (features, alpha) pairs are logged from the GPU's own Newton
   solves, the network is trained on the GPU, and learned-alpha steps are
-  checked cell by cell: every Newton-branch cell satisfies the H-residual
-  bound |G(alpha)| <= tol with G evaluated by the oracle
-  (EntropicStep.h:31-45 restated in cpu_ref.c) on the same pulled state.
-* Ablation: the same residual Newton from the stored alpha (the method
-  without the network) on the same state; the test reports what the network
-  changes in evaluations per solved cell.
+  checked cell by cell: every Newton-branch cell either satisfies the
+  H-residual bound |G(alpha)| <= tol with G evaluated by the oracle
+  (EntropicStep.h:31-45 restated in cpu_ref.c) on the same pulled state, or
+  carries the oracle's reference-Newton alpha from the same stored alpha
+  (the fallback).
+* Ablation: the same certified correction started from the stored alpha
+  (the method without the network) on the same state; the test reports what
+  the network changes in evaluations per solved cell.
 """
 import sys
 
@@ -139,23 +141,29 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
             a.iterate(1, store=True)
             res[mode] = (a.store_fields()[2], a.elbm_stats(), a.populations())
     fp, fneq = pulled_state(oracle, oc, f0)
-    newton_cells, worst = 0, 0.0
+    newton_cells, ref_fallbacks, worst, by_ref = 0, 0, 0.0, 0
     for k in range(fp.shape[1]):
-        _, b, _ = oracle.elbm_alpha(int(lat), fp[:, k], fneq[:, k], alpha0[k])
-        if b != oracle.ALPHA_NEWTON:
+        aref, b, _ = oracle.elbm_alpha(int(lat), fp[:, k], fneq[:, k], alpha0[k])
+        if b in (oracle.ALPHA_SMALL, oracle.ALPHA_CAPPED):
             continue
         newton_cells += 1
+        ref_fallbacks += b != oracle.ALPHA_NEWTON
         for mode in (AlphaMode.WARM_RESIDUAL, AlphaMode.LEARNED):
-            g = oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], res[mode][0][k])
-            worst = max(worst, abs(g))
-    say(f"{newton_cells} Newton cells, worst |G(alpha)| {worst:.3e} (tol {tol:.0e})")
+            al = res[mode][0][k]
+            g = abs(oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], al))
+            if g <= 1.001 * tol:
+                worst = max(worst, g)
+            else:  # the reference Newton decided this cell
+                assert abs(al - aref) <= 1e-9 * abs(aref), (k, mode, al, aref, g)
+                by_ref += 1
+    say(f"{newton_cells} solved cells ({ref_fallbacks} oracle fallbacks); worst accepted "
+        f"|G(alpha)| {worst:.3e} (tol {tol:.0e}); {by_ref} cell-solves left to the reference")
     assert newton_cells > 0.5 * fp.shape[1]
-    assert worst <= 1.001 * tol
     for mode, (_, st, _) in res.items():
         per = st["newton_iterations"] / max(1, st["newton"])
         say(f"mode {mode.name}: {st}; evaluations per solved cell {per:.3f}")
     st_l, st_w = res[AlphaMode.LEARNED][1], res[AlphaMode.WARM_RESIDUAL][1]
-    assert st_l["fallback"] == 0
+    assert st_l["fallback"] <= ref_fallbacks
     # all three solvers find the same root to within the residual tolerance
     fN, fL = res[AlphaMode.NEWTON][2], res[AlphaMode.LEARNED][2]
     assert np.max(np.abs(fL - fN)) <= 1e-6 * np.max(np.abs(fN))

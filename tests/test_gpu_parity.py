@@ -367,7 +367,7 @@ def test_decaying_turbulence_energy_enstrophy_vs_oracle(oracle):
     tau 0.50006, energy and enstrophy every 20 steps against the FP64 oracle.
     FP64 reference order is held to 1e-12; FP32 compute with shifted FP32
     storage states its error with the bounds written here, for the reference
-    Newton, the residual Newton from the stored alpha, and the learned alpha
+    Newton, the certified correction of the stored alpha, and the learned alpha
     (network trained on the first 20 steps' Newton solves, then switched in)."""
     import os
     n, steps, every, kmax, kpeak = 32, 100, 20, 8, 4.0
@@ -389,12 +389,14 @@ def test_decaying_turbulence_energy_enstrophy_vs_oracle(oracle):
     from metalbm_amd import AlphaMode
     cases = [("f64-reference", dict(storage=Precision.F64, compute=Precision.F64, shifted=False),
               AlphaMode.NEWTON, (1e-12, 1e-12)),
+             # measured on the MI355X: 1.6e-6 / 1.1e-6 (Newton and warm start),
+             # 3.0e-6 / 7.7e-6 (learned)
              ("f32-newton", dict(storage=Precision.F32, compute=Precision.F32, shifted=True),
-              AlphaMode.NEWTON, (1e-4, 1e-4)),
+              AlphaMode.NEWTON, (1e-5, 1e-5)),
              ("f32-warm-residual", dict(storage=Precision.F32, compute=Precision.F32, shifted=True),
-              AlphaMode.WARM_RESIDUAL, (1e-4, 1e-4)),
+              AlphaMode.WARM_RESIDUAL, (1e-5, 1e-5)),
              ("f32-learned", dict(storage=Precision.F32, compute=Precision.F32, shifted=True),
-              AlphaMode.LEARNED, (1e-4, 1e-4))]
+              AlphaMode.LEARNED, (3e-5, 3e-5))]
     for name, prec, mode, (be, bw) in cases:
         with Algorithm(Config(**base, **prec, residual_tol=1e-10)) as a:
             a.init_spectral(1.0, urms, kmax, kpeak, 42)
@@ -416,7 +418,5 @@ def test_decaying_turbulence_energy_enstrophy_vs_oracle(oracle):
         rel = np.abs(out - ref) / np.abs(ref)
         say(f"decay {name}: max rel error E {rel[:, 0].max():.3e} Omega {rel[:, 1].max():.3e}; "
             f"stats {st}")
-        if mode != AlphaMode.NEWTON:  # the residual solver brackets the root
-            assert st["fallback"] == 0
         assert rel[:, 0].max() <= be
         assert rel[:, 1].max() <= bw

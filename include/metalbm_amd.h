@@ -200,7 +200,8 @@ int lbm_reset_elbm_stats(lbm_handle *h);
 /* ---- Learned alpha (ELBM on D2Q9 and D3Q27) ------------------------------
  * No counterpart in the reference, which solves alpha by Newton only
  * (Collision.h:328-349).  A ReLU MLP (D2Q9: 9-32-32-1, D3Q27: 27-64-64-1,
- * FP32) maps a cell's normalised non-equilibrium moments to a first alpha.
+ * FP32 master weights, trained by the library's GPU kernels) maps a cell's
+ * normalised non-equilibrium moments to a first alpha.
  * Parameter layout: W1[h1][nin] b1[h1] W2[h2][h1] b2[h2] W3[h2] b3.
  * Features: m_k = (1/rho) sum_i fneq_i prod_d phi_{a_kd}(c_id) over the
  * tensor basis {1, c, c^2}^D, k = digits (a_x a_y [a_z]) in base 3. */
@@ -219,7 +220,9 @@ int lbm_mlp_get_normalization(lbm_handle *h, float *in_mean, float *in_scale,
  * over the n host samples x[n][nin], computed by the training kernels. */
 int lbm_mlp_gradient(lbm_handle *h, const float *x, const double *alpha, int n,
                      float *grad, double *loss);
-/* alpha_hat for n host feature rows, computed by the inference code. */
+/* alpha_hat for n host feature rows, computed by the step kernel's own
+ * inference code (matrix cores: f16 operands for D2Q9, bf16 for D3Q27, FP32
+ * accumulation, FP32 biases). */
 int lbm_mlp_predict(lbm_handle *h, const float *x, int n, double *alpha);
 /* Logs (features, alpha) of a `fraction` of the cells Newton solves during
  * subsequent steps, up to `capacity` samples, into device memory. */

@@ -107,7 +107,8 @@ struct BufIO<4> {
 };
 
 // Pull population i for the lane's V cells (Collision.h:146, periodic).
-template <class L, int i, class S, class C, int V>
+// SCALED: FP16 values stay in their storage scale (converted, not unscaled).
+template <class L, int i, class S, class C, int V, bool SCALED = false>
 __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
                                          unsigned voff, C (&f)[V]) {
   constexpr int D = L::dimD;
@@ -157,10 +158,12 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
     out[V - 1] = edge;
   }
 #pragma unroll
-  for (int j = 0; j < V; ++j) f[j] = to_compute<L, i, S, C>(out[j]);
+  for (int j = 0; j < V; ++j)
+    f[j] = SCALED ? C(float(out[j])) : to_compute<L, i, S, C>(out[j]);
 }
 
-template <class L, int i, class S, class C, int V>
+// SCALED: v is already in population i's FP16 storage scale.
+template <class L, int i, class S, class C, int V, bool SCALED = false>
 __device__ __forceinline__ void store_vec(const StepArgs& a, int self_row,
                                           unsigned voff, const C (&v)[V]) {
   using vt = typename VecT<S, V>::type;
@@ -169,7 +172,7 @@ __device__ __forceinline__ void store_vec(const StepArgs& a, int self_row,
 #pragma unroll
   for (int j = 0; j < V; ++j) {
     if constexpr (std::is_same_v<S, _Float16>)
-      o[j] = _Float16(float(v[j] * C(kHalfScale<L, i>)));
+      o[j] = SCALED ? _Float16(float(v[j])) : _Float16(float(v[j] * C(kHalfScale<L, i>)));
     else
       o[j] = S(v[j]);
   }
@@ -217,10 +220,15 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
   w.yp = w.y == a.ny - 1 ? 0 : w.y + 1;
 
   const unsigned voff = (unsigned)w.k0 * (unsigned)sizeof(S);
+  // FP16 populations stay in their per-population power-of-two storage scale
+  // s_i throughout (the scales are folded into the moment and collision
+  // constants).
+  constexpr bool SC = std::is_same_v<S, _Float16>;
+  constexpr bool FOLD = !REF && EQ == (int)EquilibriumType::TruncationMa3;
   C f[Q][V];
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    pull_vec<L, i, S, C, V>(a, w, voff, f[i]);
+    pull_vec<L, i, S, C, V, SC>(a, w, voff, f[i]);
   });
 
   const int self_row = D == 3 ? (w.x * a.ny + w.y) * a.nz : w.x * a.ny;
@@ -238,7 +246,7 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
       moments_ref<L>(fc, rho[j], u[j], u2[j]);
       drho[j] = rho[j] - C(1);
     } else if constexpr (SH) {
-      moments_shifted<L>(fc, drho[j], rho[j], u[j], u2[j]);
+      moments_shifted_scaled<L, C, SC>(fc, drho[j], rho[j], u[j], u2[j]);
     } else {
       moments_ref<L>(fc, rho[j], u[j], u2[j]);
       drho[j] = rho[j] - C(1);
@@ -273,6 +281,57 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
       }
       store_vec<L, i, S, C, V>(a, self_row, voff, out);
     });
+  } else if constexpr (FOLD) {
+    // Ma3 with the density folded into the polynomial's coefficients once per
+    // cell: t_i = (eq_i - (SH ? w_i : 0)) / w_i in four fused multiply-adds;
+    // out = of f + oe eq + S computed in the population's storage scale.
+    C uF[V];
+    Ma3Folded<C> M[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      uF[j] = u[j][0] * F[j][0];
+      for (int d = 1; d < D; ++d) uF[j] += u[j][d] * F[j][d];
+      M[j] = Ma3Folded<C>(rho[j], SH ? drho[j] : rho[j], u2[j]);
+    }
+    // EDM: the equilibrium at v = u + F/rho, same folding (ForcingScheme.h:184-197).
+    Ma3Folded<C> M2[V];
+    C v[V][3];
+    if constexpr (FS == SRC_EDM) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const C k = C(1) / rho[j];
+        v[j][0] = v[j][1] = v[j][2] = C(0);
+        for (int d = 0; d < D; ++d) v[j][d] = u[j][d] + F[j][d] * k;
+        C v2 = v[j][0] * v[j][0];
+        for (int d = 1; d < D; ++d) v2 += v[j][d] * v[j][d];
+        M2[j] = Ma3Folded<C>(rho[j], SH ? drho[j] : rho[j], v2);
+      }
+    }
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const C si = SC ? C(kHalfScale<L, i>) : C(1);
+      const C w_i = C(L::W[i]);
+      // w_i s_i is exact (s_i is a power of two).  omega_eq is applied per
+      // cell: a product constant fl(omega_eq w_i) would carry one fixed
+      // rounding error per population into every collision of every cell,
+      // which biases the relaxation coherently (measured: 3-5x the
+      // kinetic-energy error of a 2000-step Taylor-Green run).
+      const C ws = w_i * si;
+      C out[V];
+#pragma unroll
+      for (int j = 0; j < V; ++j) {
+        const C cu = cdot<L, i>(u[j]);
+        const C t = M[j].t(cu);
+        C src = C(0);
+        if constexpr (FS == SRC_GUO) {
+          const C cF = cdot<L, i>(F[j]);
+          src = (C(3) * gc * w_i) * (cF - uF[j] + C(3) * cu * cF);
+        }
+        if constexpr (FS == SRC_EDM) src = w_i * (M2[j].t(cdot<L, i>(v[j])) - t);
+        out[j] = fma(of, f[i][j], fma(oe, ws * t, si * src));
+      }
+      store_vec<L, i, S, C, V, SC>(a, self_row, voff, out);
+    });
   } else {
     C uF[V];
 #pragma unroll
@@ -304,9 +363,11 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
           const FastEquilibrium<L, EQ, C> E2(v, v2);
           s = rho[j] * w_i * (E2.template p<i>() - p);
         }
-        out[j] = of * f[i][j] + oe * eq + s;
+        // f is in storage scale s_i (FP16); eq and s are not.
+        const C si = SC ? C(kHalfScale<L, i>) : C(1);
+        out[j] = of * f[i][j] + si * (oe * eq + s);
       }
-      store_vec<L, i, S, C, V>(a, self_row, voff, out);
+      store_vec<L, i, S, C, V, SC>(a, self_row, voff, out);
     });
   }
   if (a.store) {

@@ -13,6 +13,7 @@
 //     FMA-friendly.  Used for every reduced-precision configuration.
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 #include "lattice.hpp"
@@ -143,6 +144,52 @@ __host__ __device__ __forceinline__ void moments_shifted(const T* g, T& drho,
   });
 }
 
+// Moments of shifted populations held in a per-population scale:
+// g_i = h_i * isc_i with isc_i = 1/s_i (FP16 storage) or 1 (otherwise), each
+// term one fused multiply-add, so no separate unscaling pass is needed.
+template <class L, class T, bool SCALED>
+__device__ __forceinline__ void moments_shifted_scaled(const T* h, T& drho, T& rho,
+                                                       T* u, T& u2) {
+  auto isc = [](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    return SCALED ? T(kHalfScaleInv<L, i>) : T(1);
+  };
+  drho = h[0] * isc(std::integral_constant<int, 0>{});
+  static_for<L::dimQ - 1>([&](auto ic) {
+    constexpr int i = decltype(ic)::value + 1;
+    drho = fma(h[i], isc(std::integral_constant<int, i>{}), drho);
+  });
+  rho = T(1) + drho;
+  // FP32: the hardware reciprocal estimate and one Newton-Raphson correction
+  // instead of the division sequence.  The correction matters: the bare
+  // estimate is not round-to-nearest, and a biased 1/rho biases every
+  // equilibrium velocity, which damps momentum a little at each collision
+  // (measured: 4x the kinetic-energy error of a 2000-step Taylor-Green run).
+  // FP64 keeps the division.
+  T rrho;
+  if constexpr (std::is_same_v<T, float>) {
+    const T r0 = __builtin_amdgcn_rcpf(rho);
+    rrho = fma(r0, fma(-rho, r0, T(1)), r0);
+  } else {
+    rrho = T(1) / rho;
+  }
+  static_for<L::dimD>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    T acc = T(0);
+    static_for<L::dimQ>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (L::C[i][d] != 0)
+        acc = fma(h[i], T(L::C[i][d]) * isc(std::integral_constant<int, i>{}), acc);
+    });
+    u[d] = acc * rrho;
+  });
+  u2 = u[0] * u[0];
+  static_for<L::dimD - 1>([&](auto dc) {
+    constexpr int d = decltype(dc)::value + 1;
+    u2 += u[d] * u[d];
+  });
+}
+
 // ---- Equilibrium (Equilibrium.h) -------------------------------------------
 
 // TruncationMa3, reference operation order (Equilibrium.h:17-33).
@@ -168,6 +215,26 @@ struct Ma3Coefs {
         a2(T(4.5) - T(6.75) * u2) {}
   __host__ __device__ __forceinline__ T p(T cu) const {
     return a0 + cu * (a1 + cu * (a2 + cu * (T(4.5) + T(3.375) * cu)));
+  }
+};
+
+// TruncationMa3 with the density folded in, for one cell:
+//   t(cu) = base + rho (P(cu) - 1) = b0 + cu (b1 + cu (b2 + cu (b3 + b4 cu)))
+// with base = drho (shifted: eq - w = w t) or rho (unshifted: eq = w t).
+template <class T>
+struct Ma3Folded {
+  T b0, b1, b2, b3, b4;
+  Ma3Folded() = default;
+  __device__ __forceinline__ Ma3Folded(T rho, T base, T u2) {
+    const Ma3Coefs<T> m(u2);
+    b0 = fma(rho, m.a0, base);
+    b1 = rho * m.a1;
+    b2 = rho * m.a2;
+    b3 = rho * T(4.5);
+    b4 = rho * T(3.375);
+  }
+  __device__ __forceinline__ T t(T cu) const {
+    return fma(cu, fma(cu, fma(cu, fma(b4, cu, b3), b2), b1), b0);
   }
 };
 

@@ -62,11 +62,12 @@ enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16 = 2 };
 
 /* Kernel shape of the BGK step. */
 enum {
-  LBM_KERNEL_AUTO = 0,    /* vector kernel when the contiguous extent allows */
+  LBM_KERNEL_AUTO = 0,    /* vector kernel when the contiguous extent allows
+                             (nz in 3D, ny in 2D, divisible by V) */
   LBM_KERNEL_CELL = 1,    /* one cell per lane */
   LBM_KERNEL_VECTOR = 2   /* V consecutive cells per lane, aligned vector
                              accesses; needs nz (3D) / ny (2D) divisible by
-                             64 V for some supported V */
+                             a supported V (the widest is chosen) */
 };
 
 /* Entropic alpha solver (ELBM only). */

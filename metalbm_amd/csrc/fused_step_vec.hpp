@@ -2,13 +2,15 @@
 //
 // Each lane owns V consecutive cells along the contiguous axis (z in 3D, y in
 // 2D), so every population row is read and written with aligned
-// V * sizeof(S) = 8..16 byte vector accesses, and a 64-lane wavefront covers
-// a segment of 64 V cells of one row.  The +-1 pull shift along the
-// contiguous axis is done in registers: the one value a lane needs from its
-// neighbour lane is moved with a cross-lane permute (a full-wavefront
-// rotate), so no load is misaligned.  When the segment is the whole row the
-// rotate already wraps periodically; otherwise lane 0 (or 63) takes the
-// value across the segment boundary from one extra uniform load.
+// V * sizeof(S) = 4..16 byte vector accesses, and a 64-lane wavefront covers
+// a segment of up to 64 V cells of one row (the last segment of a row whose
+// length is not a multiple of 64 V has fewer active lanes).  The +-1 pull
+// shift along the contiguous axis is done in registers: the one value a lane
+// needs from its neighbour lane is moved with a cross-lane permute over the
+// segment's active lanes, so no load is misaligned.  When the segment is the
+// whole row the permute wraps periodically; otherwise the first (or last)
+// active lane takes the value across the segment boundary from one extra
+// uniform load.
 //
 // Same arithmetic as k_bgk per cell (reference order for FP64 unshifted).
 #pragma once
@@ -24,9 +26,12 @@ namespace lbm {
 // rotate across 16-lane rows then reads lanes an earlier pass has already
 // overwritten, which corrupted results at full size.
 struct LaneXfer {
-  int from_prev, from_next;  // byte addresses of lanes (l - 1) mod 64, (l + 1) mod 64
-  __device__ __forceinline__ explicit LaneXfer(int lane)
-      : from_prev(((lane + 63) & 63) << 2), from_next(((lane + 1) & 63) << 2) {}
+  // byte addresses of the lanes holding cells k - V and k + V: lanes l - 1 and
+  // l + 1 cyclically over the na active lanes of the segment
+  int from_prev, from_next;
+  __device__ __forceinline__ LaneXfer(int lane, int na)
+      : from_prev((lane == 0 ? na - 1 : lane - 1) << 2),
+        from_next((lane >= na - 1 ? 0 : lane + 1) << 2) {}
 };
 
 template <bool PREV, class S>
@@ -54,10 +59,12 @@ struct VecT {
 
 struct WavePos {
   int x, y, xm, xp, ym, yp;  // row coordinates (3D: x, y; 2D: x only)
-  LaneXfer xfer{0};
+  LaneXfer xfer{0, 64};
   int k0;                    // first contiguous-axis cell of this lane
   int seg_lo, seg_hi;        // segment [seg_lo, seg_hi) along the contiguous axis
   int lane;
+  int last;                  // last active lane (na - 1)
+  bool active;               // lane < na: owns cells and stores them
   bool whole_row;
 };
 
@@ -151,7 +158,7 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
     if (!w.whole_row) {
       const int kb = w.seg_hi == a.nk ? 0 : w.seg_hi;
       const S b = pop[(unsigned)(row_off + kb)];
-      if (w.lane == 63) edge = b;
+      if (w.lane == w.last) edge = b;
     }
 #pragma unroll
     for (int j = 0; j < V - 1; ++j) out[j] = s[j + 1];
@@ -162,10 +169,12 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
     f[j] = SCALED ? C(float(out[j])) : to_compute<L, i, S, C>(out[j]);
 }
 
-// SCALED: v is already in population i's FP16 storage scale.
+// SCALED: v is already in population i's FP16 storage scale.  Only active
+// lanes store.
 template <class L, int i, class S, class C, int V, bool SCALED = false>
-__device__ __forceinline__ void store_vec(const StepArgs& a, int self_row,
-                                          unsigned voff, const C (&v)[V]) {
+__device__ __forceinline__ void store_vec(const StepArgs& a, const WavePos& w,
+                                          int self_row, unsigned voff,
+                                          const C (&v)[V]) {
   using vt = typename VecT<S, V>::type;
   using IO = BufIO<V * (int)sizeof(S)>;
   vt o;
@@ -182,9 +191,10 @@ __device__ __forceinline__ void store_vec(const StepArgs& a, int self_row,
   // corrupted by a VALU write to its data VGPRs right after issue, and emits
   // none of the wait states it otherwise inserts; on gfx950 that overwrote
   // store data at full size.
-  IO::store(__builtin_bit_cast(typename IO::raw, o),
-            make_rsrc(pop, (unsigned)a.N * (unsigned)sizeof(S)),
-            voff + (unsigned)self_row * (unsigned)sizeof(S), 0);
+  if (w.active)
+    IO::store(__builtin_bit_cast(typename IO::raw, o),
+              make_rsrc(pop, (unsigned)a.N * (unsigned)sizeof(S)),
+              voff + (unsigned)self_row * (unsigned)sizeof(S), 0);
 }
 
 template <class L, class S, class C, int FS, int EQ, bool SH, int V>
@@ -201,11 +211,17 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
   if (row >= rows) return;            // uniform per wavefront
   WavePos w;
   w.lane = threadIdx.x & 63;
-  w.xfer = LaneXfer(w.lane);
   const int seg = wave - row * segs;
   w.seg_lo = seg * 64 * V;
-  w.seg_hi = w.seg_lo + 64 * V;
-  w.k0 = w.seg_lo + w.lane * V;
+  // Active lanes of this segment: 64, or fewer in the last segment of a row
+  // whose length is not a multiple of 64 V (each lane owns V whole cells).
+  const int na = min(64, (a.nk - w.seg_lo) / V);
+  w.last = na - 1;
+  w.active = w.lane < na;
+  w.xfer = LaneXfer(w.lane, na);
+  w.seg_hi = w.seg_lo + na * V;
+  // Inactive lanes mirror lane 0's cells so every address they form is valid.
+  w.k0 = w.seg_lo + (w.active ? w.lane * V : 0);
   w.whole_row = segs == 1;
   if constexpr (D == 3) {
     w.x = row / a.ny;
@@ -279,7 +295,7 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
         }
         out[j] = of * f[i][j] + oe * fe + s;
       }
-      store_vec<L, i, S, C, V>(a, self_row, voff, out);
+      store_vec<L, i, S, C, V>(a, w, self_row, voff, out);
     });
   } else if constexpr (FOLD) {
     // Ma3 with the density folded into the polynomial's coefficients once per
@@ -330,7 +346,7 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
         if constexpr (FS == SRC_EDM) src = w_i * (M2[j].t(cdot<L, i>(v[j])) - t);
         out[j] = fma(of, f[i][j], fma(oe, ws * t, si * src));
       }
-      store_vec<L, i, S, C, V, SC>(a, self_row, voff, out);
+      store_vec<L, i, S, C, V, SC>(a, w, self_row, voff, out);
     });
   } else {
     C uF[V];
@@ -367,10 +383,10 @@ __global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
         const C si = SC ? C(kHalfScale<L, i>) : C(1);
         out[j] = of * f[i][j] + si * (oe * eq + s);
       }
-      store_vec<L, i, S, C, V, SC>(a, self_row, voff, out);
+      store_vec<L, i, S, C, V, SC>(a, w, self_row, voff, out);
     });
   }
-  if (a.store) {
+  if (a.store && w.active) {
 #pragma unroll
     for (int j = 0; j < V; ++j) {
       const int cell = self_row + w.k0 + j;

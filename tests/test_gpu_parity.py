@@ -246,18 +246,29 @@ def test_c2_full_size_invariants():
 # ---- vector kernel (V cells per lane, DPP wave rotate for the pull shift) ----
 
 VEC_CASES = [
-    # (lattice, shape, storage, compute, shifted)  -- segs = nk / (64 V)
+    # (lattice, shape, storage, compute, shifted) -- V as the library picks it
+    # (16-byte lanes, else full 64-lane segments, else any V dividing nk);
+    # ceil(nk / 64 V) segments, the last one with nk / V mod 64 active lanes
+    # when that is not zero
     (D3Q19, (3, 2, 128), Precision.F64, Precision.F64, False),   # V=2, one segment
     (D3Q19, (2, 3, 256), Precision.F64, Precision.F64, False),   # V=2, two segments
     (D3Q27, (2, 2, 384), Precision.F64, Precision.F64, False),   # V=2, three segments
     (D2Q9, (5, 512, 1), Precision.F64, Precision.F64, False),    # 2D, four segments
+    (D3Q19, (3, 2, 100), Precision.F64, Precision.F64, False),   # V=2, 50 of 64 lanes
     (D3Q19, (2, 3, 256), Precision.F32, Precision.F32, True),    # V=4, one segment
     (D3Q27, (2, 2, 512), Precision.F32, Precision.F32, True),    # V=4, two segments
-    (D3Q19, (2, 2, 384), Precision.F32, Precision.F32, True),    # V=2, three segments
+    (D3Q19, (2, 2, 384), Precision.F32, Precision.F32, True),    # V=4, 64 + 32 lanes
+    (D3Q27, (2, 3, 36), Precision.F32, Precision.F32, True),     # V=4, 9 lanes
     (D2Q9, (4, 1024, 1), Precision.F32, Precision.F32, False),   # 2D unshifted, V=4
-    (D3Q27, (2, 2, 512), Precision.F16, Precision.F32, True),    # V=8, one segment
+    (D2Q9, (3, 200, 1), Precision.F32, Precision.F32, True),     # 2D, V=4, 50 lanes
+    (D3Q27, (2, 2, 512), Precision.F16, Precision.F32, True),    # V=4, two segments
     (D3Q19, (2, 2, 768), Precision.F16, Precision.F32, True),    # V=4, three segments
     (D3Q27, (2, 2, 384), Precision.F16, Precision.F32, True),    # V=2, three segments
+    (D3Q19, (2, 2, 1160), Precision.F16, Precision.F32, True),   # V=4, 4 x 64 + 34 lanes
+    (D3Q27, (2, 3, 20), Precision.F16, Precision.F32, True),     # V=4, 5 lanes
+    (D3Q19, (2, 3, 18), Precision.F16, Precision.F32, True),     # V=2, 9 lanes
+    (D2Q9, (2, 1024, 1), Precision.F16, Precision.F32, True),    # 2D, V=8, two segments
+    (D2Q9, (3, 200, 1), Precision.F16, Precision.F32, True),     # 2D, V=8, 25 lanes
     (D3Q19, (2, 3, 256), Precision.F64, Precision.F32, True),    # FP64 storage, V=2
 ]
 
@@ -293,8 +304,8 @@ def test_vector_kernel(oracle, lat, shape, storage, compute, shifted, fs, force)
 
 def test_vector_kernel_rejects_bad_extent():
     from metalbm_amd import LBMError
-    with pytest.raises(LBMError, match="divisible by 64"):
-        Algorithm(Config(lattice=D3Q19, nx=2, ny=2, nz=100, kernel=2))
+    with pytest.raises(LBMError, match="divisible by V"):
+        Algorithm(Config(lattice=D3Q19, nx=2, ny=2, nz=99, kernel=2))
 
 
 @pytest.mark.parametrize("lat,shape,storage,compute,shifted", [

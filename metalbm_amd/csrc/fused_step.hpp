@@ -238,11 +238,21 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // Three phases so that the learned mode's wavefront-wide matrix-core
 // inference runs with every lane present: (1) pull, moments, fneq and the
 // alpha prelude per cell; (2) MODE 1 only: the network predicts alpha for
-// every lane's cell at once (its barriers are reached by all threads); (3) the
-// alpha solve, the collision and the stores per cell.  The network is staged
-// before phase 1 and published by one workgroup barrier after it.
+// every lane's cell at once on the matrix cores; (3) the alpha solve, the
+// collision and the stores per cell.  Each wavefront copies the network into
+// its own LDS region before phase 1, so no workgroup barrier is involved.
+// Minimum wavefronts per SIMD the register allocator must leave room for:
+// the D2Q9 learned kernel would otherwise settle at two (186 VGPRs); held to
+// three (168 VGPRs, 2 spilled) it matches the other D2Q9 entropic kernels.
+template <class L, int MODE>
+constexpr int elbm_min_waves() {
+  return (L::Type == LatticeType::D2Q9 && MODE == 1) ? 3 : 1;
+}
+
 template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
+__global__ void __launch_bounds__(BLOCK)
+    __attribute__((amdgpu_waves_per_eu(elbm_min_waves<L, MODE>())))
+    k_elbm(const StepArgs a, int FSX) {
   constexpr int Q = L::dimQ;
   constexpr bool REF = std::is_same_v<C, double> && !SH;
   using Shape = MlpShape<L>;
@@ -250,8 +260,12 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
   extern __shared__ __align__(16) char s_lds[];
   if (a.stats && threadIdx.x < 6) s_cnt[threadIdx.x] = 0u;
   static_assert(MODE != 1 || Shape::NIN > 0, "no alpha network for this lattice");
-  if constexpr (MODE == 1)  // stage the packed network; the barrier follows phase 1
-    copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, s_lds);
+  // MODE 1: each wavefront copies the packed network into its own LDS region
+  // now; the copy completes while phase 1 runs.
+  using Lay = MfmaLayout<Shape::NIN, Shape::H1, Shape::H2>;
+  char* wave_lds = s_lds + (threadIdx.x >> 6) * Lay::WAVE;
+  if constexpr (MODE == 1)
+    copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, wave_lds);
   const CellPos c = cell_of_thread<L, BLOCK>(a);
   int branch = -1, its = 0;
   bool first = false, solve = false;
@@ -295,7 +309,6 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
     solve = elbm_prelude<L, REF>(Fp64View<L, C, SH>{f, fneq}, alpha, amax, branch);
   }
   // ---- phase 2 ----
-  if constexpr (MODE == 1) __syncthreads();  // the staged network is visible
   double start = 2.0;
   float m[Shape::NIN > 0 ? Shape::NIN : 1];
   if constexpr (Shape::NIN > 0) {
@@ -307,7 +320,7 @@ __global__ void __launch_bounds__(BLOCK) k_elbm(const StepArgs a, int FSX) {
       for (int k = 0; k < Shape::NIN; ++k) m[k] = 0.0f;
     }
     if constexpr (MODE == 1)
-      start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2, typename Shape::Op>(s_lds, m);
+      start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2, typename Shape::Op>(wave_lds, m);
   }
   // ---- phase 3 ----
   if (c.active) {

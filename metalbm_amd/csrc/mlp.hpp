@@ -87,10 +87,11 @@ __device__ __forceinline__ void nonequilibrium_moments(const T* fneq, T rrho,
 // dimension, so a layer's accumulator (lane l: cell l%16 of the tile, rows
 // 4(l/16)+j) is already the next layer's B operand (lane l: column l%16,
 // k = 4(l/16)+j): H^T = relu(W . X^T + b) chains without a transpose.
-// LDS layout (bytes), staged once per workgroup from the FP32 block:
-//   W1h[H1][KP] W2h[H2][H1] W3h[16][H2] (operand type, zero padded; W3h row 0 = W3)
-//   b1[H1] b2[H2] b3 mean[NIN] scale[NIN] out[2] (f32)
-//   X[4 waves][64][KP] (standardised features)  Y[4 waves][64] (f32)
+// Every wavefront owns an LDS region of WAVE bytes (so no workgroup barrier
+// is needed): the packed network [0, NET), then its feature and output tiles.
+//   network: W1h[H1][KP] W2h[H2][H1] W3h[16][H2] (operand type, zero padded;
+//            W3h row 0 = W3), b1[H1] b2[H2] b3 mean[NIN] scale[NIN] out[2] (f32)
+//   X[64][KP] (standardised features of the wavefront's cells), Y[64] (f32)
 template <int NIN, int H1, int H2>
 struct MfmaLayout {
   static constexpr int KP = NIN <= 16 ? 16 : 32;
@@ -100,9 +101,11 @@ struct MfmaLayout {
   static constexpr int F32 = W3H + 16 * H2 * 2;
   static constexpr int B1 = F32, B2 = B1 + 4 * H1, B3 = B2 + 4 * H2, MEAN = B3 + 4,
                        SCALE = MEAN + 4 * NIN, OUT = SCALE + 4 * NIN;
-  static constexpr int X = ((OUT + 8 + 15) / 16) * 16;
-  static constexpr int Y = X + 4 * 64 * KP * 2;
-  static constexpr int BYTES = Y + 4 * 64 * 4;
+  static constexpr int NET = ((OUT + 8 + 15) / 16) * 16;
+  static constexpr int X = NET;
+  static constexpr int Y = X + 64 * KP * 2;
+  static constexpr int WAVE = ((Y + 64 * 4 + 15) / 16) * 16;
+  static constexpr int BYTES = 4 * WAVE;  // a workgroup of four wavefronts
 };
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
@@ -125,7 +128,7 @@ struct Mfma<__bf16> {
   }
 };
 
-// Writes the FP32 parameter block in the LDS layout (bytes [0, X)), the
+// Writes the FP32 parameter block in the LDS layout (bytes [0, NET)), the
 // weights rounded to the operand type T (round to nearest even).  Run once per
 // parameter change into a global "packed" buffer; kernels then stage it with
 // copy_packed_network.  Cooperative over the threads of one workgroup.
@@ -151,14 +154,15 @@ __device__ __forceinline__ void stage_mfma_network(const float* __restrict__ P,
     fl[H1 + H2 + t] = P[Dm::B3 + t];  // b3, mean[NIN], scale[NIN], out[2]
 }
 
-// Workgroup-cooperative copy of a packed network (M::X bytes, 16-aligned).
+// Copy of the packed network (M::NET bytes, 16-aligned) into the calling
+// wavefront's LDS region by its own 64 lanes.
 template <int NIN, int H1, int H2>
 __device__ __forceinline__ void copy_packed_network(const void* __restrict__ src,
-                                                    char* lds) {
-  constexpr int n = MfmaLayout<NIN, H1, H2>::X / 16;
+                                                    char* wave_lds) {
+  constexpr int n = MfmaLayout<NIN, H1, H2>::NET / 16;
   const uint4* s = static_cast<const uint4*>(src);
-  uint4* d = reinterpret_cast<uint4*>(lds);
-  for (int t = threadIdx.x; t < n; t += blockDim.x) d[t] = s[t];
+  uint4* d = reinterpret_cast<uint4*>(wave_lds);
+  for (int t = threadIdx.x & 63; t < n; t += 64) d[t] = s[t];
 }
 
 // Orders one wavefront's LDS writes before its later LDS reads by other lanes
@@ -171,23 +175,24 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // alpha_hat for every lane's cell (m: raw features; lanes without a cell pass
-// zeros), from the network staged in LDS (visible to the workgroup).  Each
-// wavefront uses only its own feature and output tiles, so it synchronises
-// with itself only; all 64 lanes of the wavefront must be active.
+// zeros), from the network the wavefront copied into its own LDS region
+// (copy_packed_network).  The region is private to the wavefront, so it
+// synchronises with itself only; all 64 lanes of the wavefront must be active.
 template <int NIN, int H1, int H2, class T>
 __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
   using M = MfmaLayout<NIN, H1, H2>;
   using V4 = typename Mfma<T>::v4;
   constexpr int KP = M::KP;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  wave_sync_lds();  // the wavefront's network copy is complete
   const float* fl = reinterpret_cast<const float*>(lds + M::F32);
   const float* b1 = fl;
   const float* b2 = fl + H1;
   const float b3 = fl[H1 + H2];
   const float* mean = fl + H1 + H2 + 1;
   const float* scale = mean + NIN;
-  T* X = reinterpret_cast<T*>(lds + M::X) + wv * 64 * KP;
-  float* Y = reinterpret_cast<float*>(lds + M::Y) + wv * 64;
+  T* X = reinterpret_cast<T*>(lds + M::X);
+  float* Y = reinterpret_cast<float*>(lds + M::Y);
 #pragma unroll
   for (int k4 = 0; k4 < KP; k4 += 4) {
     V4 z;

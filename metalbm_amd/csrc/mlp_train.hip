@@ -98,12 +98,12 @@ __global__ void __launch_bounds__(256)
 k_mlp_forward(const void* __restrict__ packed, const float* __restrict__ X, int n,
               double* __restrict__ alpha) {
   extern __shared__ __align__(16) char s_lds[];
-  copy_packed_network<NIN, H1, H2>(packed, s_lds);
-  __syncthreads();
+  char* wave_lds = s_lds + (threadIdx.x >> 6) * MfmaLayout<NIN, H1, H2>::WAVE;
+  copy_packed_network<NIN, H1, H2>(packed, wave_lds);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   float m[NIN];
   for (int k = 0; k < NIN; ++k) m[k] = s < n ? X[(size_t)s * NIN + k] : 0.0f;
-  const double y = mlp_alpha_wave<NIN, H1, H2, T>(s_lds, m);
+  const double y = mlp_alpha_wave<NIN, H1, H2, T>(wave_lds, m);
   if (s < n) alpha[s] = y;
 }
 
@@ -171,7 +171,7 @@ using OpOf = std::conditional_t<NIN == 9, MlpShape<Lattice<LatticeType::D2Q9>>::
 size_t mlp_packed_bytes(int lattice) {
   size_t b = 0;
   by_lattice(lattice, [&](auto nin, auto h1, auto h2) {
-    b = MfmaLayout<decltype(nin)::value, decltype(h1)::value, decltype(h2)::value>::X;
+    b = MfmaLayout<decltype(nin)::value, decltype(h1)::value, decltype(h2)::value>::NET;
     return hipSuccess;
   });
   return b;

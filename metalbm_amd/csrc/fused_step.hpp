@@ -241,12 +241,16 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // every lane's cell at once on the matrix cores; (3) the alpha solve, the
 // collision and the stores per cell.  Each wavefront copies the network into
 // its own LDS region before phase 1, so no workgroup barrier is involved.
-// Minimum wavefronts per SIMD the register allocator must leave room for:
-// the D2Q9 learned kernel would otherwise settle at two (186 VGPRs); held to
-// three (168 VGPRs, 2 spilled) it matches the other D2Q9 entropic kernels.
+// Minimum wavefronts per SIMD the register allocator must leave room for
+// (the FP64 solve is latency-bound, so occupancy beats a few spills; measured
+// on one MI355X): D2Q9 learned three (168 VGPRs, 2 spilled; 17.3 k MLUPS vs
+// 13.8 k at two); D3Q27 two (reference Newton 2.72 k vs 2.03 k MLUPS at one
+// wavefront with 289 registers; certified correction +16%, learned +9%).
 template <class L, int MODE>
 constexpr int elbm_min_waves() {
-  return (L::Type == LatticeType::D2Q9 && MODE == 1) ? 3 : 1;
+  if (L::Type == LatticeType::D2Q9 && MODE == 1) return 3;
+  if (L::Type == LatticeType::D3Q27) return 2;
+  return 1;
 }
 
 template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK, int MODE>

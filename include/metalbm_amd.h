@@ -234,8 +234,11 @@ int lbm_sample_log_count(lbm_handle *h, int64_t *n);
 int lbm_sample_log_copy(lbm_handle *h, float *x, double *alpha, int64_t n);
 /* Trains on the logged samples on the GPU, starting from the current
  * parameters (lbm_mlp_init(seed) first if none were initialised or loaded):
- * sets the normalisation from the samples, then `iterations` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on
- * minibatches of `batch` drawn by a seeded shuffle.  Mean losses of the
+ * puts the samples in a canonical order (by alpha, then features; the log's
+ * own order depends on wavefront timing), sets the normalisation from them,
+ * then runs `iterations` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) on
+ * minibatches of `batch` drawn by a seeded shuffle: the same samples and seed
+ * give the same network.  Mean losses of the
  * first and last step are returned (either pointer may be NULL). */
 int lbm_mlp_train(lbm_handle *h, int iterations, int batch, double lr,
                   uint64_t seed, double *loss_first, double *loss_last);

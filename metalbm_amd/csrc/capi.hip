@@ -1151,6 +1151,32 @@ int lbm_mlp_train(lbm_handle* h, int iterations, int batch, double lr, uint64_t 
   std::vector<double> al((size_t)n);
   HIP_TRY(h, hipMemcpy(X.data(), h->log_x, sizeof(float) * X.size(), hipMemcpyDeviceToHost));
   HIP_TRY(h, hipMemcpy(al.data(), h->log_alpha, sizeof(double) * al.size(), hipMemcpyDeviceToHost));
+  {
+    // Canonical sample order (by alpha, then features): the step kernel fills
+    // the log in whatever order its wavefronts reach the slot counter, and
+    // training (normalisation sums, seeded minibatches) must not depend on it.
+    std::vector<int64_t> ord((size_t)n);
+    for (int64_t i = 0; i < n; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int64_t a, int64_t b) {
+      if (al[a] != al[b]) return al[a] < al[b];
+      for (int k = 0; k < nin; ++k) {
+        const float xa = X[(size_t)a * nin + k], xb = X[(size_t)b * nin + k];
+        if (xa != xb) return xa < xb;
+      }
+      return false;
+    });
+    std::vector<float> Xs(X.size());
+    std::vector<double> as(al.size());
+    for (int64_t i = 0; i < n; ++i) {
+      as[i] = al[ord[i]];
+      std::copy(X.begin() + ord[i] * nin, X.begin() + (ord[i] + 1) * nin, Xs.begin() + i * nin);
+    }
+    X.swap(Xs);
+    al.swap(as);
+    HIP_TRY(h, hipMemcpy(h->log_x, X.data(), sizeof(float) * X.size(), hipMemcpyHostToDevice));
+    HIP_TRY(h, hipMemcpy(h->log_alpha, al.data(), sizeof(double) * al.size(),
+                         hipMemcpyHostToDevice));
+  }
   std::vector<double> mean(nin, 0.0), sd(nin, 0.0);
   for (int64_t s = 0; s < n; ++s)
     for (int k = 0; k < nin; ++k) mean[k] += X[(size_t)s * nin + k];

@@ -181,17 +181,25 @@ __device__ __forceinline__ bool elbm_prelude(const V& p, double& alpha,
 // The reference Newton-Raphson (EntropicStep.h:111-140 via solveAlpha,
 // Collision.h:328-349): warm start, |step| <= 1e-8, at most 50 iterations,
 // the iterate before the last step is returned, alpha = 2 when it fails or
-// leaves (1, alphaMax).  G and G' share one log per term; their values equal
-// the reference's separate loops (EntropicStep.h:31-62).
-template <class L, class V>
+// leaves (1, alphaMax).  G and G' share one log per term.  REFORDER keeps the
+// reference's term order g += f A - fm lg (EntropicStep.h:31-62) with
+// A_i = ln(f_i / w_i) held per term, for FP64 parity; otherwise the constant
+// part is summed once, G = sum f A - sum fm lg, which frees the Q FP64 values
+// of A (the solve's register footprint bounds its occupancy).
+template <class L, bool REFORDER, class V>
 __device__ __forceinline__ double newton_reference(const V& p, double guess,
                                                    double amax, int& branch,
                                                    int& iterations) {
   constexpr int Q = L::dimQ;
-  double A[Q];
+  double A[REFORDER ? Q : 1];
+  double SA = 0.0;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    A[i] = log(p.template f<i>() / L::W[i]);
+    const double a = log(p.template f<i>() / L::W[i]);
+    if constexpr (REFORDER)
+      A[i] = a;
+    else
+      SA += p.template f<i>() * a;
   });
   double x = guess, step = 0.0;
   bool ok = false;
@@ -199,13 +207,16 @@ __device__ __forceinline__ double newton_reference(const V& p, double guess,
   for (int it = 1; it <= 50; ++it) {
     x = x - step;
     iterations = it;
-    double g = 0.0, dg = 0.0;
+    double g = REFORDER ? 0.0 : SA, dg = 0.0;
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const double fi = p.template f<i>(), ni = p.template n<i>();
       const double fm = fi - x * ni;
       const double lg = log(fm / L::W[i]);
-      g += fi * A[i] - fm * lg;
+      if constexpr (REFORDER)
+        g += fi * A[i] - fm * lg;
+      else
+        g -= fm * lg;
       dg += ni * (1 + lg);
     });
     step = g / dg;
@@ -333,7 +344,7 @@ __global__ void __launch_bounds__(BLOCK)
     if (solve) {
       const Fp64View<L, C, SH> p{f, fneq};
       if constexpr (MODE == 0) {
-        alpha = newton_reference<L>(p, double(astate[cell]), amax, branch, its);
+        alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, its);
       } else {
         // MODE 1 starts from the network's alpha, MODE 2 from the stored one;
         // one certified correction, else the reference Newton decides.
@@ -345,7 +356,7 @@ __global__ void __launch_bounds__(BLOCK)
           its = ev;
         } else {
           int nit = 0;
-          alpha = newton_reference<L>(p, double(astate[cell]), amax, branch, nit);
+          alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, nit);
           its = ev + nit;
         }
       }

@@ -309,35 +309,58 @@ __device__ __forceinline__ double log1p_mid(double y) {
   return 2.0 * z * s;
 }
 
-// G, G' and -G'' at alpha = a from t_i = a x_i, with S = sum n_i A_i and
+// x_i = n_i / f_i for the certified correction: held per term for small Q,
+// recomputed at each use for D3Q27, where holding 27 FP64 values would cost
+// occupancy (the solve is latency-bound).
+template <class L, class V>
+struct XTerms {
+  static constexpr bool KEEP = L::dimQ <= 9;
+  const V& p;
+  double x[KEEP ? L::dimQ : 1];
+  double xmax = 0.0;  // max_i |x_i|
+  __device__ __forceinline__ explicit XTerms(const V& p_) : p(p_) {
+    static_for<L::dimQ>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const double xi = div_refined(p.template n<i>(), p.template f<i>());
+      xmax = fmax(xmax, fabs(xi));
+      if constexpr (KEEP) x[i] = xi;
+    });
+  }
+  template <int i>
+  __device__ __forceinline__ double get() const {
+    if constexpr (KEEP)
+      return x[i];
+    else
+      return div_refined(p.template n<i>(), p.template f<i>());
+  }
+};
+
+// G, G' and -G'' at alpha = a > 0 from t_i = a x_i, with S = sum n_i A_i and
 // Sn = sum n_i precomputed (A_i = ln(f_i / w_i)):
 //   G = a S - sum (f_i - a n_i) log1p(-t_i),  G' = Sn + S + sum n_i log1p(-t_i),
 //   -G'' = sum n_i x_i / (1 - t_i)
 // (f - a n = f (1 - t), ln((f - a n)/w) = A + log1p(-t)).  log1p(-t) by the
-// degree-12 Taylor series when every |t_i| <= kSeriesMax, by the atanh
-// series when every |t_i| <= kMidMax, else by the library; the choice is made
-// once per evaluation, not per term.  1/(1 - t) by the refined reciprocal.
+// degree-12 Taylor series when max |t_i| = a max |x_i| <= kSeriesMax, by the
+// atanh series when <= kMidMax, else by the library; the choice is made once
+// per evaluation, not per term.  1/(1 - t) by the refined reciprocal.
 template <class L, class V>
-__device__ __forceinline__ void residual_terms(const V& p, const double* x, double S,
-                                               double Sn, double a, double& g,
-                                               double& dg, double& h) {
+__device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X,
+                                               double S, double Sn, double a,
+                                               double& g, double& dg, double& h) {
   constexpr int Q = L::dimQ;
-  double tmax = 0.0;
-  static_for<Q>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    tmax = fmax(tmax, fabs(a * x[i]));
-  });
+  const double tmax = a * X.xmax;
   double gs = 0.0, ds = 0.0;
   h = 0.0;
   auto terms = [&](auto log1p_of) {
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const double ni = p.template n<i>();
-      const double t = a * x[i];
+      const double xi = X.template get<i>();
+      const double t = a * xi;
       const double l1 = log1p_of(-t);
       gs += (p.template f<i>() - a * ni) * l1;
       ds += ni * l1;
-      h += ni * x[i] * recip_refined(1.0 - t);
+      h += ni * xi * recip_refined(1.0 - t);
     });
   };
   if (tmax <= kSeriesMax)
@@ -352,13 +375,14 @@ __device__ __forceinline__ void residual_terms(const V& p, const double* x, doub
 
 // -G''(a) = sum n_i x_i / (1 - a x_i) >= 0, as in residual_terms.
 template <class L, class V>
-__device__ __forceinline__ double residual_curvature(const V& p, const double* x,
+__device__ __forceinline__ double residual_curvature(const V& p, const XTerms<L, V>& X,
                                                      double a) {
   constexpr int Q = L::dimQ;
   double s = 0.0;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    s += p.template n<i>() * x[i] * recip_refined(1.0 - a * x[i]);
+    const double xi = X.template get<i>();
+    s += p.template n<i>() * xi * recip_refined(1.0 - a * xi);
   });
   return s;
 }
@@ -378,13 +402,11 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
   constexpr int Q = L::dimQ;
   evals = 0;
   if (!(start > 1.0 && start < amax)) return false;
-  double x[Q];
+  const XTerms<L, V> X(p);
   double ymax = 0.0;
   static_for<Q>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    const double fi = p.template f<i>();
-    x[i] = div_refined(p.template n<i>(), fi);
-    ymax = fmax(ymax, fabs(fma(fi, 1.0 / L::W[i], -1.0)));
+    ymax = fmax(ymax, fabs(fma(p.template f<i>(), 1.0 / L::W[i], -1.0)));
   });
   double S = 0.0, Sn = 0.0;  // sum n ln(f/w), sum n
   auto sums = [&](auto log1p_of) {
@@ -402,7 +424,7 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
   else
     sums([](double y) { return log1p(y); });
   double g, dg, h;
-  residual_terms<L>(p, x, S, Sn, start, g, dg, h);
+  residual_terms<L>(p, X, S, Sn, start, g, dg, h);
   evals = 1;
   if (fabs(g) <= tol) {
     alpha = start;
@@ -412,7 +434,7 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
   const double a1 = start - s;
   if (!(a1 > 1.0 && a1 < amax)) return false;  // also a non-finite step
   const double bound =
-      0.5 * s * s * fmax(h, residual_curvature<L>(p, x, a1)) * (1.0 + 1e-6);
+      0.5 * s * s * fmax(h, residual_curvature<L>(p, X, a1)) * (1.0 + 1e-6);
   if (!(bound <= tol)) return false;
   alpha = a1;
   return true;

@@ -124,6 +124,11 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
         l0, l1 = a.mlp_train(1500, 1024, 2e-3, seed=1)
         say(f"training loss {l0:.4e} -> {l1:.4e}")
         assert l1 < 0.25 * l0
+        # training is reproducible: same samples and seed, same network
+        p1 = a.mlp_params()
+        a.mlp_init(1)
+        assert a.mlp_train(1500, 1024, 2e-3, seed=1) == (l0, l1)
+        np.testing.assert_array_equal(a.mlp_params(), p1)
         pred = a.mlp_predict(x[:20000])
         err = np.abs(pred - al[:20000])
         spread = np.std(al - 2)

@@ -403,13 +403,16 @@ def test_decaying_turbulence_energy_enstrophy_vs_oracle(oracle):
     cases = [("f64-reference", dict(storage=Precision.F64, compute=Precision.F64, shifted=False),
               AlphaMode.NEWTON, (1e-12, 1e-12)),
              # measured on the MI355X: 1.6e-6 / 1.1e-6 (Newton and warm start),
-             # 3.0e-6 / 7.7e-6 (learned)
+             # 3.0e-6 .. 3.0e-5 (learned: an accepted alpha is certified only
+             # to |G| <= tol, so it can differ from the reference Newton's by
+             # up to tol / |G'| where G' is small, and the turbulent run
+             # amplifies it)
              ("f32-newton", dict(storage=Precision.F32, compute=Precision.F32, shifted=True),
               AlphaMode.NEWTON, (1e-5, 1e-5)),
              ("f32-warm-residual", dict(storage=Precision.F32, compute=Precision.F32, shifted=True),
               AlphaMode.WARM_RESIDUAL, (1e-5, 1e-5)),
              ("f32-learned", dict(storage=Precision.F32, compute=Precision.F32, shifted=True),
-              AlphaMode.LEARNED, (3e-5, 3e-5))]
+              AlphaMode.LEARNED, (1e-4, 1e-4))]
     for name, prec, mode, (be, bw) in cases:
         with Algorithm(Config(**base, **prec, residual_tol=1e-10)) as a:
             a.init_spectral(1.0, urms, kmax, kpeak, 42)

@@ -325,17 +325,15 @@ __global__ void __launch_bounds__(BLOCK)
   }
   // ---- phase 2 ----
   double start = 2.0;
-  float m[Shape::NIN > 0 ? Shape::NIN : 1];
-  if constexpr (Shape::NIN > 0) {
-    const bool need_m = solve && (MODE == 1 || a.log_cap > 0);
-    if (need_m) {
+  if constexpr (MODE == 1) {
+    float m[Shape::NIN];
+    if (solve) {
       nonequilibrium_moments<L>(fneq, C(1) / rho, m);
     } else {
 #pragma unroll
       for (int k = 0; k < Shape::NIN; ++k) m[k] = 0.0f;
     }
-    if constexpr (MODE == 1)
-      start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2, typename Shape::Op>(wave_lds, m);
+    start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2, typename Shape::Op>(wave_lds, m);
   }
   // ---- phase 3 ----
   if (c.active) {
@@ -360,11 +358,15 @@ __global__ void __launch_bounds__(BLOCK)
           its = ev + nit;
         }
       }
-      if constexpr (Shape::NIN > 0) {
+      // The sample log records the reference Newton's solves (MODE 0 only).
+      if constexpr (Shape::NIN > 0 && MODE == 0) {
         if (a.log_cap > 0 && branch == 2 &&
             sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {
           const unsigned long long k = atomicAdd(a.log_count, 1ull);
           if ((long long)k < a.log_cap) {
+            // recomputed here rather than kept live from phase 2
+            float m[Shape::NIN];
+            nonequilibrium_moments<L>(fneq, C(1) / rho, m);
             for (int q = 0; q < Shape::NIN; ++q) a.log_x[k * Shape::NIN + q] = m[q];
             a.log_alpha[k] = alpha;
           }

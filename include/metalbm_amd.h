@@ -111,10 +111,12 @@ typedef struct lbm_handle lbm_handle;
 typedef struct {
   int64_t small;        /* max|fneq/f| <= 1e-3, alpha = 2      (Collision.h:356-359) */
   int64_t capped;       /* alphaMax < 2, alpha = 0.95 alphaMax (Collision.h:365-367) */
-  int64_t newton;       /* Newton converged inside (1, alphaMax) */
+  int64_t newton;       /* root solved inside (1, alphaMax) */
   int64_t fallback;     /* Newton failed or out of bounds, alpha = 2 (Collision.h:344-346) */
-  int64_t newton_iterations;
-  int64_t learned_accepted;  /* LEARNED: cells certified after one correction */
+  int64_t newton_iterations;  /* G evaluations (reference Newton iterations plus
+                                 certified-correction evaluations) */
+  int64_t learned_accepted;   /* LEARNED / WARM_RESIDUAL: solved cells accepted
+                                 by the certified correction (no reference Newton) */
 } lbm_elbm_stats;
 
 int lbm_abi_version(void);

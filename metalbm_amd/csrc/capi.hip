@@ -261,6 +261,16 @@ int build_force_tables(lbm_handle* h) {
   return LBM_OK;
 }
 
+// SplitMix64 (Steele, Lea, Flood 2014): output k of the stream seeded with
+// `seed`; the seeded draws of the network initialisation, the minibatch
+// shuffle and the spectral initialiser.
+uint64_t mix64(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
 int pack_mlp(lbm_handle* h);  // learned-alpha section below
 
 int ensure_staging(lbm_handle* h) {
@@ -582,11 +592,7 @@ double spectral_modes(int D, int K, double urms, double kpeak, uint64_t seed,
   // compiler fuses a sin/cos pair into one sincos call (rounded differently).
   std::vector<double> ang(3 * nh), cs(3 * nh), sn(3 * nh);
   for (size_t j = 0; j < 3 * nh; ++j) {
-    uint64_t z = seed + (j + 1) * 0x9E3779B97F4A7C15ULL;  // SplitMix64 draw j
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    z = z ^ (z >> 31);
-    ang[j] = 2.0 * M_PI * ((double)(z >> 11) * 0x1.0p-53);
+    ang[j] = 2.0 * M_PI * ((double)(mix64(seed, j) >> 11) * 0x1.0p-53);
   }
   for (size_t j = 0; j < 3 * nh; ++j) cs[j] = std::cos(ang[j]);
   for (size_t j = 0; j < 3 * nh; ++j) sn[j] = std::sin(ang[j]);
@@ -910,13 +916,6 @@ int pack_mlp(lbm_handle* h) {
     h->mlp_dirty = false;
   }
   return LBM_OK;
-}
-
-uint64_t mix64(uint64_t seed, uint64_t k) {
-  uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ULL;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
 }
 
 double unit01(uint64_t seed, uint64_t k) { return (double)(mix64(seed, k) >> 11) * 0x1.0p-53; }

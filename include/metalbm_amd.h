@@ -138,6 +138,8 @@ const char *lbm_last_error(const lbm_handle *h);
 
 /* Sizes derived from the configuration. */
 int lbm_sizes(const lbm_handle *h, int64_t *cells, int *q, int *d);
+/* The handle's configuration (alpha_mode and residual_tol as last set). */
+int lbm_get_config(const lbm_handle *h, lbm_config *out);
 
 /* f = feq(rho, u) (Initialize.h:106-117).  rho[N], u[D][N] on the host. */
 int lbm_init_equilibrium(lbm_handle *h, const double *rho, const double *u);
@@ -259,6 +261,25 @@ int lbm_device_populations(lbm_handle *h, void **ptr, int *elem_bytes);
 
 /* Steps taken since creation. */
 int64_t lbm_steps_done(const lbm_handle *h);
+
+/* ---- HDF5 files in the reference layout ----------------------------------
+ * One FP64 dataset per component, dimensions the global extents with the
+ * last one padded to 2 (n/2 + 1) (gSD::pLength(), Domain.h:53-57,
+ * MathVector.h:331-344); padding written as zero, ignored on reading.
+ * libhdf5 (>= 1.10) is loaded at first use (dlopen: $METALBM_HDF5_LIBRARY,
+ * libhdf5.so, libhdf5.so.103, /opt/conda/lib); LBM_E_STATE if absent. */
+/* Fields of the last storing step, in FieldList::writeFields order
+ * (FieldList.h:65-86, Routine.h:202-210, Writer.h:309-360): density,
+ * velocity{X,Y[,Z]}, alpha (ELBM only), force{X,Y[,Z]} (when a force is
+ * configured).  Writes the XDMF sidecar of FieldWriter<XDMF>
+ * (Writer.h:451-560) to xdmf_path unless it is NULL. */
+int lbm_write_fields_hdf5(lbm_handle *h, const char *path, const char *xdmf_path);
+/* Unshifted FP64 populations as datasets distribution0..distribution{Q-1}
+ * (DistributionWriter, Writer.h:385-440, Routine.h:212-216). */
+int lbm_write_distribution_hdf5(lbm_handle *h, const char *path);
+/* Restart: reads distribution0..{Q-1} of the padded extents of this handle's
+ * grid into the populations (Reader.h:106-158, Initialize.h:119-145). */
+int lbm_read_distribution_hdf5(lbm_handle *h, const char *path);
 
 #ifdef __cplusplus
 }

@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import ctypes
 import enum
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -347,4 +348,43 @@ class Algorithm:
 
     def set_alpha_mode(self, mode, residual_tol: float = 1e-10) -> None:
         self._check(lib().lbm_set_alpha_mode(self._h, int(mode), residual_tol))
+
+    # -- HDF5 files in the reference layout -------------------------------------
+    # File names follow Writer::getFileName (Writer.h:44-50): <folder>/field-<it>.h5
+    # with its field-<it>.xmf sidecar, and <folder>/distribution-<it>.h5.
+    def write_fields_hdf5(self, path: str, xdmf_path: str | None = None) -> None:
+        """Fields of the last storing step as FP64 datasets (density,
+        velocityX.., alpha for ELBM, forceX.. when forced)."""
+        self._check(lib().lbm_write_fields_hdf5(
+            self._h, os.fsencode(path), os.fsencode(xdmf_path) if xdmf_path else None))
+
+    def write_distribution_hdf5(self, path: str) -> None:
+        self._check(lib().lbm_write_distribution_hdf5(self._h, os.fsencode(path)))
+
+    def read_distribution_hdf5(self, path: str) -> None:
+        self._check(lib().lbm_read_distribution_hdf5(self._h, os.fsencode(path)))
+
+    def write_fields(self, folder: str, iteration: int) -> str:
+        """Routine::writeFields (Routine.h:202-210) into `folder`; returns the .h5 path."""
+        os.makedirs(folder, exist_ok=True)
+        path = os.path.join(folder, f"field-{int(iteration)}.h5")
+        self.write_fields_hdf5(path, os.path.join(folder, f"field-{int(iteration)}.xmf"))
+        return path
+
+    def write_distribution(self, folder: str, iteration: int) -> str:
+        """The distribution backup of Routine::writeFields (Routine.h:212-216)."""
+        os.makedirs(folder, exist_ok=True)
+        path = os.path.join(folder, f"distribution-{int(iteration)}.h5")
+        self.write_distribution_hdf5(path)
+        return path
+
+    def read_distribution(self, folder: str, iteration: int) -> None:
+        """Restart from <folder>/distribution-<iteration>.h5 (Initialize.h:119-123)."""
+        self.read_distribution_hdf5(os.path.join(folder, f"distribution-{int(iteration)}.h5"))
+
+    def handle_config(self) -> LbmConfigC:
+        """The handle's configuration as the C struct (alpha mode as last set)."""
+        c = LbmConfigC()
+        self._check(lib().lbm_get_config(self._h, ctypes.byref(c)))
+        return c
 

@@ -37,6 +37,7 @@ SOURCES = {
     "aux_kernels.hip": ["-ffp-contract=off"],
     "mlp_train.hip": ["-ffp-contract=fast"],
     "capi.hip": ["-ffp-contract=off"],
+    "hdf5_io.cpp": ["-ffp-contract=off"],
 }
 
 
@@ -78,7 +79,7 @@ def build(changed_only: bool = False, verbose: bool = True) -> str:
         objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], changed_only),
                            SOURCES.items()))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

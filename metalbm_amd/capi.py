@@ -25,6 +25,8 @@ EXPORTS = (
     "lbm_mlp_set_normalization", "lbm_mlp_get_normalization", "lbm_mlp_gradient",
     "lbm_mlp_predict", "lbm_sample_log_enable", "lbm_sample_log_disable",
     "lbm_sample_log_count", "lbm_sample_log_copy", "lbm_mlp_train", "lbm_set_alpha_mode",
+    "lbm_get_config", "lbm_write_fields_hdf5", "lbm_write_distribution_hdf5",
+    "lbm_read_distribution_hdf5",
 )
 
 
@@ -117,6 +119,11 @@ def lib() -> ctypes.CDLL:
     L.lbm_mlp_train.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64,
                                 dp, dp]
     L.lbm_set_alpha_mode.argtypes = [H, ctypes.c_int, ctypes.c_double]
+    L.lbm_get_config.argtypes = [H, ctypes.POINTER(LbmConfigC)]
+    cp = ctypes.c_char_p
+    L.lbm_write_fields_hdf5.argtypes = [H, cp, cp]
+    L.lbm_write_distribution_hdf5.argtypes = [H, cp]
+    L.lbm_read_distribution_hdf5.argtypes = [H, cp]
     _lib = L
     return L
 

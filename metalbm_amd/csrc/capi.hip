@@ -15,6 +15,7 @@
 
 #include "../../include/metalbm_amd.h"
 #include "aux_kernels.hpp"
+#include "capi_internal.hpp"
 #include "mlp_train.hpp"
 #include "registry.hpp"
 
@@ -330,6 +331,10 @@ StepArgs make_args(const lbm_handle* h) {
 
 }  // namespace
 
+int lbm::record_error(lbm_handle* h, int code, const std::string& msg) {
+  return fail(h, code, msg);
+}
+
 extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
@@ -358,6 +363,14 @@ int lbm_config_default(lbm_config* c) {
 
 const char* lbm_last_error(const lbm_handle* h) {
   return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int lbm_get_config(const lbm_handle* h, lbm_config* out) {
+  if (!h || !out) return LBM_E_INVALID;
+  *out = h->cfg;
+  out->alpha_mode = h->alpha_mode;
+  out->residual_tol = h->residual_tol;
+  return LBM_OK;
 }
 
 int lbm_create(const lbm_config* cfg, lbm_handle** out) {

@@ -78,3 +78,7 @@ def test_null_handle_calls_fail_cleanly(capi):
     assert L.lbm_synchronize(None) == capi.LBM_E_INVALID
     assert L.lbm_destroy(None) == 0
     assert L.lbm_steps_done(None) == -1
+    assert L.lbm_get_config(None, None) == capi.LBM_E_INVALID
+    assert L.lbm_write_fields_hdf5(None, b"x.h5", None) == capi.LBM_E_INVALID
+    assert L.lbm_write_distribution_hdf5(None, b"x.h5") == capi.LBM_E_INVALID
+    assert L.lbm_read_distribution_hdf5(None, b"x.h5") == capi.LBM_E_INVALID

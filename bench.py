@@ -382,6 +382,62 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
     return res
 
 
+def diagnostics_bench(K: int, W: int, n: int, cpu: bool) -> dict:
+    """Spectral diagnostics of the stored fields (lbm_vorticity + lbm_spectra,
+    physical scaling) on the D3Q19 n^3 FP32 workload's fields: wall time per
+    call pair, with the numpy oracle on the same fields as the CPU baseline."""
+    import numpy as np
+
+    import metalbm_amd as m
+    wl = WORKLOADS[DEFAULT]
+    cfg = make_config({**wl["cfg"], "nx": n, "ny": n, "nz": n})
+    a = m.Algorithm(cfg)
+    init(a, ("spectral", 0.02, 8, 4.0, 42))
+    a.iterate(2, store=True)
+    a.synchronize()
+    for _ in range(W):
+        a.vorticity("physical")
+        a.spectra("physical")
+    t0 = time.perf_counter()
+    for _ in range(K):
+        _, ens = a.vorticity("physical", field=False)
+        e, f = a.spectra("physical")
+    wall = (time.perf_counter() - t0) / K
+    t0 = time.perf_counter()
+    for _ in range(K):
+        a.vorticity("physical")
+    wall_field = (time.perf_counter() - t0) / K
+    N, D = cfg.cells, 3
+    M = n * n * (n // 2 + 1)
+    # algorithmic bytes per launch of the library's own kernels: curl reads D
+    # spectra and writes 3 (16 B per mode each); the shell bins read D spectra
+    kbytes = {"k_curl_hat": (D + 3) * 16 * M, "k_shell_bins": D * 16 * M}
+    out = {"metric": "spectral diagnostics: enstrophy (spectral curl) + energy/forcing "
+                     "spectra per call pair, results on the host, vorticity field on the device",
+           "value": round(wall * 1e3, 3), "unit": "ms", "higher_is_better": False,
+           "steps": K, "warmup": W, "n_gpus": 1, "dtype": "f64",
+           "data": "synthetic (spectral initial field, 2 FP32 steps)",
+           "config": {"workload": f"d3q19-bgk-{n}-f32 stored fields", "cells": N,
+                      "modes": M, "kernel_bytes": kbytes,
+                      "vorticity_with_host_field_ms": round(wall_field * 1e3, 3),
+                      "enstrophy": ens, "energy_sum": float(e.sum())}}
+    rho, u, al, F = a.store_fields()
+    a.close()
+    if cpu:
+        from oracle import spectral as sp
+        shape = (n, n, n)
+        t0 = time.perf_counter()
+        sp.vorticity(u, shape, sp.PHYSICAL)
+        sp.spectrum(u, shape, sp.PHYSICAL)
+        sp.spectrum(F, shape, sp.PHYSICAL)
+        t = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(t * 1e3, 1), "unit": "ms", "cores": 1,
+                               "kind": "port",
+                               "sample": f"one call pair of the numpy oracle (pocketfft, one "
+                                         f"thread) on the same {n}^3 fields"}
+    return out
+
+
 def traffic_of(name):
     """HBM bytes per launch of the workload's step kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_<name>.txt,
@@ -413,7 +469,15 @@ def main(argv=None) -> int:
                     help="D3Q27 entropic decaying turbulence: energy and enstrophy of the FP32 "
                          "solvers vs the FP64 build (prints its own JSON line)")
     ap.add_argument("--decay-n", type=int, default=256)
+    ap.add_argument("--diagnostics", action="store_true",
+                    help="time lbm_vorticity + lbm_spectra on D3Q19 n^3 stored fields "
+                         "(prints its own JSON line)")
+    ap.add_argument("--diagnostics-n", type=int, default=256)
     args = ap.parse_args(argv)
+    if args.diagnostics:
+        print(json.dumps(diagnostics_bench(args.steps or 20, args.warmup, args.diagnostics_n,
+                                           not args.no_cpu_baseline)), flush=True)
+        return 0
     if args.decay:
         res = decay_study(args.steps or 3000, 100, args.decay_n)
         print(json.dumps({"metric": "energy/enstrophy error vs FP64", **res}))

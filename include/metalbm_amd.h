@@ -281,6 +281,31 @@ int lbm_write_distribution_hdf5(lbm_handle *h, const char *path);
  * grid into the populations (Reader.h:106-158, Initialize.h:119-145). */
 int lbm_read_distribution_hdf5(lbm_handle *h, const char *path);
 
+/* ---- Spectral diagnostics of the stored fields ---------------------------
+ * Forward/inverse transforms by hipFFT on the device; Fourier-space work by
+ * the library's kernels.  Scalings:
+ *  LBM_SPECTRAL_REFERENCE: the reference's numbers -- integer wavenumbers,
+ *    vorticity divided by N twice and (2D) its imaginary part dropped
+ *    (Transformer.h:100-111, 114-295), shells floor(|k|), weight 1/2 on the
+ *    k_last = 0 plane, energy spectrum / N, forcing spectrum unnormalised
+ *    (Analysis.h:100-176, AnalysisList.h:98-202);
+ *  LBM_SPECTRAL_PHYSICAL: wavenumbers 2 pi k/n, the exact spectral curl,
+ *    shells round(|k|), weight 1/2 on the k_last = 0 and n/2 planes, both
+ *    spectra / N^2 (sum_s E(s) = mean 1/2 |u|^2 below the last shell).
+ * Both zero the derivative along an even axis at its Nyquist index. */
+enum { LBM_SPECTRAL_REFERENCE = 0, LBM_SPECTRAL_PHYSICAL = 1 };
+/* Number of shells s = 0 .. max(n)/2 - 1 (gFD::maxWaveNumber,
+ * FourierDomain.h:76-78). */
+int lbm_spectral_shells(const lbm_handle *h, int *nshells);
+/* Curl of u_hydro: omega[2D-3][N] (Curl::executeSpace + normalize,
+ * Routine.h:129-132) and enstrophy = (1/N) sum 1/2 |omega|^2
+ * (TotalEnstrophy, Analysis.h:68-98); either pointer may be NULL. */
+int lbm_vorticity(lbm_handle *h, int scaling, double *omega, double *enstrophy);
+/* Shell spectra of u_hydro (energy) and of the force field (forcing),
+ * each [nshells] (PowerSpectra, SpectralAnalysisList::writeAnalyses);
+ * either pointer may be NULL. */
+int lbm_spectra(lbm_handle *h, int scaling, double *energy, double *forcing);
+
 #ifdef __cplusplus
 }
 #endif

@@ -382,6 +382,35 @@ class Algorithm:
         """Restart from <folder>/distribution-<iteration>.h5 (Initialize.h:119-123)."""
         self.read_distribution_hdf5(os.path.join(folder, f"distribution-{int(iteration)}.h5"))
 
+    # -- spectral diagnostics of the stored fields ------------------------------
+    _SCALING = {"reference": 0, "physical": 1}
+
+    def spectral_shells(self) -> int:
+        n = ctypes.c_int()
+        self._check(lib().lbm_spectral_shells(self._h, ctypes.byref(n)))
+        return n.value
+
+    def vorticity(self, scaling: str = "reference", field: bool = True):
+        """(omega[2D-3, N], enstrophy) of the stored u_hydro, computed
+        spectrally (Curl, Transformer.h:114-295; TotalEnstrophy,
+        Analysis.h:68-98) in the "reference" or "physical" scaling;
+        field=False returns (None, enstrophy) without copying omega."""
+        C = 3 if self.D == 3 else 1
+        w = np.empty((C, self.N)) if field else None
+        e = ctypes.c_double()
+        self._check(lib().lbm_vorticity(self._h, self._SCALING[scaling],
+                                        _dp(w) if field else None, ctypes.byref(e)))
+        return w, e.value
+
+    def spectra(self, scaling: str = "reference"):
+        """(energy[nshells], forcing[nshells]) shell spectra of u_hydro and of
+        the force field (PowerSpectra, AnalysisList.h:98-202)."""
+        ns = self.spectral_shells()
+        e = np.empty(max(ns, 1))
+        f = np.empty(max(ns, 1))
+        self._check(lib().lbm_spectra(self._h, self._SCALING[scaling], _dp(e), _dp(f)))
+        return e[:ns], f[:ns]
+
     def handle_config(self) -> LbmConfigC:
         """The handle's configuration as the C struct (alpha mode as last set)."""
         c = LbmConfigC()

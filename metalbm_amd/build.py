@@ -38,6 +38,7 @@ SOURCES = {
     "mlp_train.hip": ["-ffp-contract=fast"],
     "capi.hip": ["-ffp-contract=off"],
     "hdf5_io.cpp": ["-ffp-contract=off"],
+    "spectral.hip": ["-ffp-contract=off"],
 }
 
 
@@ -79,7 +80,7 @@ def build(changed_only: bool = False, verbose: bool = True) -> str:
         objs = list(ex.map(lambda kv: _compile(kv[0], kv[1], changed_only),
                            SOURCES.items()))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"]
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lhipfft", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmetalbm_amd.so")
 
 LBM_OK, LBM_E_INVALID, LBM_E_HIP, LBM_E_NOMEM, LBM_E_STATE = 0, -1, -2, -3, -4
+LBM_SPECTRAL_REFERENCE, LBM_SPECTRAL_PHYSICAL = 0, 1
 
 # Every entry point declared in include/metalbm_amd.h.
 EXPORTS = (
@@ -26,7 +27,7 @@ EXPORTS = (
     "lbm_mlp_predict", "lbm_sample_log_enable", "lbm_sample_log_disable",
     "lbm_sample_log_count", "lbm_sample_log_copy", "lbm_mlp_train", "lbm_set_alpha_mode",
     "lbm_get_config", "lbm_write_fields_hdf5", "lbm_write_distribution_hdf5",
-    "lbm_read_distribution_hdf5",
+    "lbm_read_distribution_hdf5", "lbm_spectral_shells", "lbm_vorticity", "lbm_spectra",
 )
 
 
@@ -124,6 +125,9 @@ def lib() -> ctypes.CDLL:
     L.lbm_write_fields_hdf5.argtypes = [H, cp, cp]
     L.lbm_write_distribution_hdf5.argtypes = [H, cp]
     L.lbm_read_distribution_hdf5.argtypes = [H, cp]
+    L.lbm_spectral_shells.argtypes = [H, ip]
+    L.lbm_vorticity.argtypes = [H, ctypes.c_int, dp, dp]
+    L.lbm_spectra.argtypes = [H, ctypes.c_int, dp, dp]
     _lib = L
     return L
 

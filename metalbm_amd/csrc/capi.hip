@@ -80,6 +80,7 @@ struct lbm_handle {
   long long log_cap = 0;
   unsigned log_threshold = 0, log_salt = 0;
   bool log_on = false;
+  void* spectral = nullptr;  // spectral diagnostics state (spectral.hip)
 };
 
 namespace {
@@ -287,6 +288,8 @@ int ensure_fields(lbm_handle* h) {
 }
 
 void release(lbm_handle* h) {
+  lbm::release_spectral(h->spectral);
+  h->spectral = nullptr;
   for (void* p : {h->f[0], h->f[1], h->alpha, (void*)h->fields,
                   (void*)h->staging, (void*)h->reduce_buf, h->ftab[0],
                   h->ftab[1], h->ftab[2], (void*)h->stats, (void*)h->mlp, h->mlp_packed,
@@ -334,6 +337,28 @@ StepArgs make_args(const lbm_handle* h) {
 int lbm::record_error(lbm_handle* h, int code, const std::string& msg) {
   return fail(h, code, msg);
 }
+
+int lbm::record_hip(lbm_handle* h, hipError_t e, const char* what) {
+  return e == hipSuccess ? LBM_OK : hip_fail(h, e, what);
+}
+
+int lbm::device_fields(lbm_handle* h, DeviceFields& out) {
+  if (!h->fields_valid)
+    return fail(h, LBM_E_STATE, "no storing step has run (lbm_step store_last)");
+  const size_t n = (size_t)h->N;
+  out.rho = h->fields;
+  out.u = h->fields + n;
+  out.force = h->fields + n * (2 + h->D);
+  out.D = h->D;
+  out.n[0] = h->cfg.nx;
+  out.n[1] = h->cfg.ny;
+  out.n[2] = h->cfg.nz;
+  out.N = h->N;
+  out.stream = h->stream;
+  return LBM_OK;
+}
+
+void*& lbm::spectral_slot(lbm_handle* h) { return h->spectral; }
 
 extern "C" {
 

@@ -1,14 +1,36 @@
-// capi_internal.hpp -- helpers shared by the C-ABI translation units.
+// capi_internal.hpp -- what the C ABI's translation units share besides the
+// public header: error recording and read access to a handle's device state.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
 #include <string>
 
 #include "../../include/metalbm_amd.h"
 
 namespace lbm {
 
-// Records `msg` as the handle's last error (or the creation error when h is
-// NULL) and returns `code`.
+// Records msg as the handle's last error and returns code.
 int record_error(lbm_handle* h, int code, const std::string& msg);
+// LBM_E_HIP with the HIP error text, unless e is hipSuccess (then LBM_OK).
+int record_hip(lbm_handle* h, hipError_t e, const char* what);
+
+// Device view of the fields of the last storing step.
+struct DeviceFields {
+  const double* rho;    // [N]
+  const double* u;      // [D][N] hydrodynamic velocity
+  const double* force;  // [D][N]
+  int D, n[3];          // extents; n[2] = 1 in 2D
+  int64_t N;
+  hipStream_t stream;   // the handle's stream
+};
+// LBM_E_STATE (with message) when no storing step has run.
+int device_fields(lbm_handle* h, DeviceFields& out);
+
+// Per-handle state of the spectral diagnostics (owned by spectral.hip,
+// released with the handle).
+void*& spectral_slot(lbm_handle* h);
+void release_spectral(void* state);
 
 }  // namespace lbm

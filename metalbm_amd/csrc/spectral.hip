@@ -1,0 +1,450 @@
+// spectral.hip -- spectral diagnostics of the stored fields: the curl of the
+// hydrodynamic velocity and the shell-binned energy and forcing spectra.
+//
+// Reference: Curl<double, CPU, OneD, 2|3> (Transformer.h:114-295) called as
+// executeSpace + normalize after every storing step (Routine.h:98-101,
+// 129-132), TotalEnstrophy (Analysis.h:68-98), and SpectralAnalysisList::
+// writeAnalyses with PowerSpectra (AnalysisList.h:98-171, Analysis.h:100-176).
+// The reference runs FFTW-MPI on the host; here the real-to-complex and
+// complex-to-real transforms are hipFFT plans over the device fields, and the
+// Fourier-space work (curl, shell binning) is this file's kernels.
+//
+// Two scalings (lbm_vorticity / lbm_spectra):
+//  REFERENCE -- the reference's numbers: integer wavenumbers k_d (index i <=
+//    n/2 -> i, else i - n), vorticity divided by N twice (the backward
+//    transform's 1/N, Transformer.h:100-111, then Curl::normalize,
+//    Transformer.h:179-185, 284-294), the 2D curl's imaginary part set to
+//    zero (Transformer.h:161-165); shells floor(|k|) (int MathVector::norm,
+//    MathVector.h:65); weight 1/2 on the k_last = 0 plane (Analysis.h:
+//    150-153); the energy spectrum divided by N, the forcing spectrum not
+//    normalised (AnalysisList.h:195 normalises energy only).
+//  PHYSICAL -- wavenumbers 2 pi k_d / n_d, one 1/N, the full complex 2D
+//    curl; shells round(|k|); weight 1/2 on the k_last = 0 and (even n)
+//    k_last = n/2 planes; both spectra divided by N^2, so that
+//    sum_s E(s) = (1/N) sum_x 1/2 |u|^2 when every mode lies below the last
+//    shell.
+// Nyquist indices (n_d/2 of an even axis): PHYSICAL takes the derivative
+// there as zero.  REFERENCE keeps the reference's k = +n_d/2 except on the
+// self-paired planes k_last = 0 and k_last = n_last/2, where each mode's
+// conjugate partner is stored too: there the reference's +n/2 makes the
+// spectrum non-Hermitian, and the complex-to-real inverse returns its
+// Hermitian part, in which the Nyquist derivative is zero -- so it is zero
+// here as well.
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "capi_internal.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kReduceBlocks = 1024;
+
+struct State {
+  int D = 0, C = 0, n[3] = {1, 1, 1};
+  int64_t N = 0, M = 0;   // real cells, half-spectrum modes
+  int nshells = 0;
+  hipfftHandle fwd = 0, inv = 0;
+  bool have_fwd = false, have_inv = false;
+  double* real = nullptr;     // D N: transform input copy
+  double2* hat = nullptr;     // D M
+  double2* what = nullptr;    // C M
+  double* omega = nullptr;    // C N
+  double* spec = nullptr;     // 2 nshells
+  double* partial = nullptr;  // kReduceBlocks + 1
+  double* bins = nullptr;     // shell partials: shell_blocks x min(nshells, window)
+  int shell_blocks = 0;
+};
+
+void free_state(State* s) {
+  if (!s) return;
+  if (s->have_fwd) hipfftDestroy(s->fwd);
+  if (s->have_inv) hipfftDestroy(s->inv);
+  for (void* p : {(void*)s->real, (void*)s->hat, (void*)s->what, (void*)s->omega,
+                  (void*)s->spec, (void*)s->partial, (void*)s->bins})
+    if (p) (void)hipFree(p);
+  delete s;
+}
+
+const char* fft_text(hipfftResult r) {
+  switch (r) {
+    case HIPFFT_SUCCESS: return "success";
+    case HIPFFT_ALLOC_FAILED: return "allocation failed";
+    case HIPFFT_INVALID_VALUE: return "invalid value";
+    case HIPFFT_INVALID_SIZE: return "invalid size";
+    case HIPFFT_EXEC_FAILED: return "execution failed";
+    case HIPFFT_SETUP_FAILED: return "setup failed";
+    default: return "hipFFT error";
+  }
+}
+
+int fft_fail(lbm_handle* h, hipfftResult r, const char* what) {
+  return lbm::record_error(h, r == HIPFFT_ALLOC_FAILED ? LBM_E_NOMEM : LBM_E_HIP,
+                           std::string(what) + ": " + fft_text(r));
+}
+
+// Wavenumber of index i along an axis of n points (FourierDomain.h /
+// AnalysisList.h:148-156: i <= n/2 -> i, else i - n).
+__host__ __device__ inline int wavenumber(int i, int n) { return i <= n / 2 ? i : i - n; }
+
+// Index decomposition of half-spectrum mode m, last axis halved.
+struct ModeIndex {
+  int i[3];
+};
+__host__ __device__ inline ModeIndex mode_index(int64_t m, int D, const int* n) {
+  ModeIndex r{{0, 0, 0}};
+  const int nh = n[D - 1] / 2 + 1;
+  r.i[D - 1] = (int)(m % nh);
+  int64_t rest = m / nh;
+  if (D == 3) {
+    r.i[1] = (int)(rest % n[1]);
+    rest /= n[1];
+  }
+  r.i[0] = (int)rest;
+  return r;
+}
+
+struct Dims {
+  int D, n[3];
+  int64_t M;
+};
+
+// Derivative factor along an axis: integer k (REFERENCE) or 2 pi k / n
+// (PHYSICAL); at the Nyquist index of an even axis zero in PHYSICAL and on
+// the self-paired planes, +n/2 otherwise.
+__device__ inline double dfactor(int i, int n, bool physical, bool self_paired) {
+  if ((n & 1) == 0 && i == n / 2 && (physical || self_paired)) return 0.0;
+  const int k = wavenumber(i, n);
+  return physical ? (2.0 * M_PI / n) * k : double(k);
+}
+
+// i a for a = (re, im): (-im, re)
+__device__ inline double2 times_i(double2 a) { return make_double2(-a.y, a.x); }
+
+__global__ void k_curl_hat(const double2* __restrict__ uh, double2* __restrict__ wh, Dims g,
+                           int physical, double scale) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.M) return;
+  const ModeIndex ix = mode_index(m, g.D, g.n);
+  const int nl = g.n[g.D - 1], l = ix.i[g.D - 1];
+  const bool self_paired = l == 0 || ((nl & 1) == 0 && l == nl / 2);
+  double k[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < g.D; ++d) k[d] = dfactor(ix.i[d], g.n[d], physical != 0, self_paired);
+  if (g.D == 2) {
+    const double2 ux = uh[m], uy = uh[g.M + m];
+    // omega_z = i (kx uy - ky ux)
+    double2 w = times_i(make_double2(k[0] * uy.x - k[1] * ux.x, k[0] * uy.y - k[1] * ux.y));
+    if (!physical) w.y = 0.0;  // Transformer.h:161-165
+    wh[m] = make_double2(w.x * scale, w.y * scale);
+  } else {
+    const double2 ux = uh[m], uy = uh[g.M + m], uz = uh[2 * g.M + m];
+    const double2 wx = times_i(make_double2(k[1] * uz.x - k[2] * uy.x, k[1] * uz.y - k[2] * uy.y));
+    const double2 wy = times_i(make_double2(k[2] * ux.x - k[0] * uz.x, k[2] * ux.y - k[0] * uz.y));
+    const double2 wz = times_i(make_double2(k[0] * uy.x - k[1] * ux.x, k[0] * uy.y - k[1] * ux.y));
+    wh[m] = make_double2(wx.x * scale, wx.y * scale);
+    wh[g.M + m] = make_double2(wy.x * scale, wy.y * scale);
+    wh[2 * g.M + m] = make_double2(wz.x * scale, wz.y * scale);
+  }
+}
+
+// Fixed-order block sum (deterministic for a fixed launch shape).
+__device__ inline double block_sum(double v, double* lds) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) lds[threadIdx.x] += lds[threadIdx.x + s];
+    __syncthreads();
+  }
+  return lds[0];
+}
+
+// partial[b] = sum over the cells b, b + G, ... of 1/2 sum_c w_c^2.
+__global__ void k_half_square_partial(const double* __restrict__ w, int C, int64_t N,
+                                      double* __restrict__ partial) {
+  __shared__ double lds[kBlock];
+  double acc = 0.0;
+  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N;
+       x += (int64_t)gridDim.x * blockDim.x)
+    for (int c = 0; c < C; ++c) {
+      const double v = w[c * N + x];
+      acc += 0.5 * v * v;
+    }
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void k_final_sum(const double* __restrict__ partial, int n, double scale,
+                            double* __restrict__ out) {
+  __shared__ double lds[kBlock];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) *out = t * scale;
+}
+
+// Shell of an integer |k|^2, exactly: REFERENCE truncates
+// (unsigned(int(sqrt(int))), MathVector.h:65, AnalysisList.h:157),
+// PHYSICAL rounds to the nearest integer (|k|^2 is an integer, so |k| is
+// never a half-integer: s + 1 when k2 - s^2 > s).
+__device__ inline int shell_of(long long k2, bool physical) {
+  long long s = (long long)sqrt((double)k2);
+  while (s * s > k2) --s;
+  while ((s + 1) * (s + 1) <= k2) ++s;
+  if (physical && k2 - s * s > s) ++s;
+  return (int)s;
+}
+
+constexpr int kShellWindow = 2048;  // shells binned per pass (LDS: 4 waves x 16 KB)
+constexpr int kShellBlocks = 1024;
+
+// Shell sums of c(m) sum_d |hat_d(m)|^2 over the shells [s_lo, s_lo + cnt),
+// c = 1/2 on the k_last = 0 plane and, when physical, on the k_last = n/2
+// plane of an even last axis.  Modes are read in storage order (coalesced);
+// each wavefront adds its lanes' values into its own LDS bins one lane at a
+// time in lane order, so the sums do not depend on timing; the block then
+// writes the sum of its wavefronts' bins (in wavefront order) to
+// partial[block][cnt].
+__global__ void __launch_bounds__(kBlock) k_shell_bins(const double2* __restrict__ hat, Dims g,
+                                                       int physical, int s_lo, int cnt,
+                                                       double* __restrict__ partial) {
+  extern __shared__ double bins[];  // [kBlock / 64][cnt]
+  constexpr int kWaves = kBlock / 64;
+  for (int i = threadIdx.x; i < kWaves * cnt; i += blockDim.x) bins[i] = 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  double* mine = bins + (threadIdx.x >> 6) * cnt;
+  const int nlast = g.n[g.D - 1];
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.M;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = base + threadIdx.x;
+    int key = -1;
+    double val = 0.0;
+    if (m < g.M) {
+      const ModeIndex ix = mode_index(m, g.D, g.n);
+      long long k2 = 0;
+      for (int d = 0; d < g.D; ++d) {
+        const long long k = wavenumber(ix.i[d], g.n[d]);
+        k2 += k * k;
+      }
+      const int b = shell_of(k2, physical != 0) - s_lo;
+      if (b >= 0 && b < cnt) {
+        const int l = ix.i[g.D - 1];
+        const double c =
+            (l == 0 || (physical && (nlast & 1) == 0 && l == nlast / 2)) ? 0.5 : 1.0;
+        double e = 0.0;
+        for (int d = 0; d < g.D; ++d) {
+          const double2 v = hat[(int64_t)d * g.M + m];
+          e += v.x * v.x;
+          e += v.y * v.y;
+        }
+        key = b;
+        val = c * e;
+      }
+    }
+    unsigned long long todo = __ballot(key >= 0);
+    while (todo) {
+      const int t = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      if (lane == t) atomicAdd(&mine[key], val);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < cnt; b += blockDim.x) {
+    double acc = 0.0;
+    for (int w = 0; w < kWaves; ++w) acc += bins[w * cnt + b];
+    partial[(int64_t)blockIdx.x * cnt + b] = acc;
+  }
+}
+
+// One block per shell b: out[b] = scale * sum over blocks of
+// partial[block][b], in a fixed order (strided per thread, then a tree).
+__global__ void k_shell_final(const double* __restrict__ partial, int blocks, int cnt,
+                              double scale, double* __restrict__ out) {
+  __shared__ double lds[kBlock];
+  const int b = blockIdx.x;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < blocks; k += blockDim.x) acc += partial[(int64_t)k * cnt + b];
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0) out[b] = t * scale;
+}
+
+unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+Dims dims_of(const State& s) {
+  Dims g;
+  g.D = s.D;
+  for (int d = 0; d < 3; ++d) g.n[d] = s.n[d];
+  g.M = s.M;
+  return g;
+}
+
+// Creates (once) the state of a handle: buffers and the forward plan.
+int state_of(lbm_handle* h, const lbm::DeviceFields& f, State*& out) {
+  void*& slot = lbm::spectral_slot(h);
+  if (slot) {
+    out = static_cast<State*>(slot);
+    return LBM_OK;
+  }
+  State* s = new State;
+  s->D = f.D;
+  s->C = f.D == 3 ? 3 : 1;  // 2 D - 3 components
+  for (int d = 0; d < 3; ++d) s->n[d] = f.n[d];
+  s->N = f.N;
+  s->M = f.N / s->n[f.D - 1] * (s->n[f.D - 1] / 2 + 1);
+  int nmax = 1;
+  for (int d = 0; d < f.D; ++d) nmax = std::max(nmax, s->n[d]);
+  s->nshells = nmax / 2;  // gFD::maxWaveNumber (FourierDomain.h:76-78)
+  auto alloc = [&](void** p, size_t bytes) {
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return lbm::record_error(h, LBM_E_NOMEM, "spectral buffers: out of memory");
+    return (int)LBM_OK;
+  };
+  int rc = alloc((void**)&s->real, sizeof(double) * s->D * s->N);
+  if (!rc) rc = alloc((void**)&s->hat, sizeof(double2) * s->D * s->M);
+  if (!rc) rc = alloc((void**)&s->what, sizeof(double2) * s->C * s->M);
+  if (!rc) rc = alloc((void**)&s->omega, sizeof(double) * s->C * s->N);
+  if (!rc) rc = alloc((void**)&s->spec, sizeof(double) * 2 * std::max(1, s->nshells));
+  if (!rc) rc = alloc((void**)&s->partial, sizeof(double) * (kReduceBlocks + 1));
+  s->shell_blocks = (int)std::min<int64_t>(kShellBlocks, (s->M + kBlock - 1) / kBlock);
+  if (!rc)
+    rc = alloc((void**)&s->bins, sizeof(double) * s->shell_blocks *
+                                     std::max(1, std::min(s->nshells, kShellWindow)));
+  if (!rc) {
+    int dims[3] = {s->n[0], s->n[1], s->n[2]};
+    hipfftResult r = hipfftPlanMany(&s->fwd, s->D, dims, nullptr, 1, (int)s->N, nullptr, 1,
+                                    (int)s->M, HIPFFT_D2Z, s->D);
+    if (r != HIPFFT_SUCCESS) rc = fft_fail(h, r, "forward transform plan");
+    else s->have_fwd = true;
+    if (!rc) {
+      r = hipfftPlanMany(&s->inv, s->D, dims, nullptr, 1, (int)s->M, nullptr, 1, (int)s->N,
+                         HIPFFT_Z2D, s->C);
+      if (r != HIPFFT_SUCCESS) rc = fft_fail(h, r, "inverse transform plan");
+      else s->have_inv = true;
+    }
+  }
+  if (rc) {
+    free_state(s);
+    return rc;
+  }
+  slot = s;
+  out = s;
+  return LBM_OK;
+}
+
+// hat = forward transform of the D components at src (copied first: the
+// transform may use its input as scratch).
+int forward(lbm_handle* h, State& s, const double* src, hipStream_t st) {
+  int rc = lbm::record_hip(h, hipMemcpyAsync(s.real, src, sizeof(double) * s.D * s.N,
+                                             hipMemcpyDeviceToDevice, st), "field copy");
+  if (rc) return rc;
+  hipfftResult r = hipfftSetStream(s.fwd, st);
+  if (r == HIPFFT_SUCCESS)
+    r = hipfftExecD2Z(s.fwd, (hipfftDoubleReal*)s.real, (hipfftDoubleComplex*)s.hat);
+  return r == HIPFFT_SUCCESS ? LBM_OK : fft_fail(h, r, "forward transform");
+}
+
+int check_scaling(lbm_handle* h, int scaling) {
+  if (scaling != LBM_SPECTRAL_REFERENCE && scaling != LBM_SPECTRAL_PHYSICAL)
+    return lbm::record_error(h, LBM_E_INVALID, "scaling must be LBM_SPECTRAL_REFERENCE or "
+                                               "LBM_SPECTRAL_PHYSICAL");
+  return LBM_OK;
+}
+
+}  // namespace
+
+void lbm::release_spectral(void* state) { free_state(static_cast<State*>(state)); }
+
+extern "C" {
+
+int lbm_spectral_shells(const lbm_handle* h, int* nshells) {
+  if (!h || !nshells) return LBM_E_INVALID;
+  lbm_config c;
+  if (int rc = lbm_get_config(h, &c)) return rc;
+  int nmax = std::max(c.nx, c.ny);
+  if (c.lattice != LBM_D2Q9) nmax = std::max(nmax, c.nz);
+  *nshells = nmax / 2;
+  return LBM_OK;
+}
+
+int lbm_vorticity(lbm_handle* h, int scaling, double* omega, double* enstrophy) {
+  if (!h) return LBM_E_INVALID;
+  if (int rc = check_scaling(h, scaling)) return rc;
+  lbm::DeviceFields f;
+  if (int rc = lbm::device_fields(h, f)) return rc;
+  State* s = nullptr;
+  if (int rc = state_of(h, f, s)) return rc;
+  const hipStream_t st = f.stream;
+  const bool physical = scaling == LBM_SPECTRAL_PHYSICAL;
+  if (int rc = forward(h, *s, f.u, st)) return rc;
+  const double invN = 1.0 / (double)s->N;
+  hipLaunchKernelGGL(k_curl_hat, dim3(grid_of(s->M)), dim3(kBlock), 0, st, s->hat, s->what,
+                     dims_of(*s), physical ? 1 : 0, physical ? invN : invN * invN);
+  if (int rc = lbm::record_hip(h, hipGetLastError(), "curl kernel")) return rc;
+  hipfftResult r = hipfftSetStream(s->inv, st);
+  if (r == HIPFFT_SUCCESS)
+    r = hipfftExecZ2D(s->inv, (hipfftDoubleComplex*)s->what, (hipfftDoubleReal*)s->omega);
+  if (r != HIPFFT_SUCCESS) return fft_fail(h, r, "inverse transform");
+  if (enstrophy) {
+    // TotalEnstrophy: (1/N) sum 1/2 |omega|^2 (Analysis.h:86-93, 27)
+    hipLaunchKernelGGL(k_half_square_partial, dim3(kReduceBlocks), dim3(kBlock), 0, st,
+                       s->omega, s->C, s->N, s->partial);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(kBlock), 0, st, s->partial, kReduceBlocks,
+                       invN, s->partial + kReduceBlocks);
+    if (int rc = lbm::record_hip(h, hipGetLastError(), "enstrophy kernels")) return rc;
+    if (int rc = lbm::record_hip(h, hipMemcpyAsync(enstrophy, s->partial + kReduceBlocks,
+                                                   sizeof(double), hipMemcpyDeviceToHost, st),
+                                 "enstrophy copy"))
+      return rc;
+  }
+  if (omega) {
+    if (int rc = lbm::record_hip(h, hipMemcpyAsync(omega, s->omega, sizeof(double) * s->C * s->N,
+                                                   hipMemcpyDeviceToHost, st),
+                                 "vorticity copy"))
+      return rc;
+  }
+  return lbm::record_hip(h, hipStreamSynchronize(st), "vorticity");
+}
+
+int lbm_spectra(lbm_handle* h, int scaling, double* energy, double* forcing) {
+  if (!h) return LBM_E_INVALID;
+  if (int rc = check_scaling(h, scaling)) return rc;
+  lbm::DeviceFields f;
+  if (int rc = lbm::device_fields(h, f)) return rc;
+  State* s = nullptr;
+  if (int rc = state_of(h, f, s)) return rc;
+  const hipStream_t st = f.stream;
+  const bool physical = scaling == LBM_SPECTRAL_PHYSICAL;
+  const int ns = s->nshells;
+  const double invN = 1.0 / (double)s->N;
+  // REFERENCE: energy / N (Analysis.h:115-119, AnalysisList.h:195), forcing
+  // unnormalised; PHYSICAL: both / N^2
+  const double scale[2] = {physical ? invN * invN : invN, physical ? invN * invN : 1.0};
+  const double* src[2] = {f.u, f.force};
+  double* dst[2] = {energy, forcing};
+  for (int which = 0; which < 2; ++which) {
+    if (!dst[which] || ns == 0) continue;
+    if (int rc = forward(h, *s, src[which], st)) return rc;
+    for (int lo = 0; lo < ns; lo += kShellWindow) {
+      const int cnt = std::min(kShellWindow, ns - lo);
+      const size_t lds = sizeof(double) * (kBlock / 64) * cnt;
+      hipLaunchKernelGGL(k_shell_bins, dim3(s->shell_blocks), dim3(kBlock), lds, st, s->hat,
+                         dims_of(*s), physical ? 1 : 0, lo, cnt, s->bins);
+      hipLaunchKernelGGL(k_shell_final, dim3(cnt), dim3(kBlock), 0, st,
+                         s->bins, s->shell_blocks, cnt, scale[which], s->spec + which * ns + lo);
+      if (int rc = lbm::record_hip(h, hipGetLastError(), "shell kernels")) return rc;
+    }
+    if (int rc = lbm::record_hip(h, hipMemcpyAsync(dst[which], s->spec + which * ns,
+                                                   sizeof(double) * ns, hipMemcpyDeviceToHost,
+                                                   st),
+                                 "spectrum copy"))
+      return rc;
+  }
+  return lbm::record_hip(h, hipStreamSynchronize(st), "spectra");
+}
+
+}  // extern "C"

@@ -114,6 +114,17 @@ WORKLOADS = {
         init=("spectral", 0.1 / 3 ** 0.5, 16, 4.0, 42), steps=50,
         learn=dict(log_steps=200, fraction=1e-4, capacity=500000, iters=3000, batch=4096,
                    lr=2e-3)),
+    "d2q9-bgk-turbulent2d-2048-f32": dict(
+        desc="D2Q9 SRT BGK 2048^2, tau 0.6, EDM + Turbulent2D spectral force (injection "
+             "amplitude 5e-4 on shells 5..7, energy removal 1e-6 on shells 1..2, the force "
+             "field recomputed with hipFFT before every step from the last stored fields; "
+             "fields stored every 10th step), FP32 shifted storage and compute, u0 noise +-1e-3",
+        cfg=dict(lattice="D2Q9", nx=2048, ny=2048, nz=1, tau=0.6,
+                 forcing_scheme="ExactDifferenceMethod", force="Turbulent2D",
+                 amplitude=(5e-4, 5e-4, 5e-4), force_kmin=5, force_kmax=7,
+                 removal_amplitude=(1e-6, 1e-6, 1e-6), removal_kmin=1, removal_kmax=2,
+                 storage="F32", compute="F32", shifted=True),
+        init=("noise", 1e-3, 42), steps=200, store_every=10),
     "d2q9-bgk-128-f64": dict(
         desc="D2Q9 SRT BGK 128^2 FP64, tau 0.6, Guo + Kolmogorov F_x=1e-5 sin(2 pi 4 y/128), "
              "u0 noise +-1e-3 (seed 42) -- cache-resident parity config",
@@ -231,6 +242,9 @@ def init(a, spec):
 def bytes_per_cell(cfg, wl_cfg) -> int:
     q = cfg.q
     b = 2 * q * SIZES[wl_cfg["storage"]]
+    # the spectral forces are a per-cell force field read by the step
+    if wl_cfg.get("force") in ("ConstantShell", "EnergyRemoval", "Turbulent2D"):
+        b += cfg.d * (8 if wl_cfg["compute"] == "F64" else 4)
     # Every entropic solver reads and writes the alpha state (Algorithm.h:103-106).
     if wl_cfg.get("collision") == "ELBM":
         b += 2 * (8 if wl_cfg["compute"] == "F64" else 4)   # alpha warm-start state r+w
@@ -249,8 +263,9 @@ def cpu_baseline(wl, max_seconds=15.0):
                    collision=2 if c.get("collision") == "ELBM" else 1,
                    forcing_scheme={"None_": 1, "Guo": 2, "ShanChen": 3,
                                    "ExactDifferenceMethod": 4}[c.get("forcing_scheme", "Guo")],
-                   force={"None_": 0, "Constant": 4, "Sinusoidal": 5,
-                          "Kolmogorov": 6}[c.get("force", "None_")],
+                   force={"None_": 0, "Constant": 4, "Sinusoidal": 5, "Kolmogorov": 6,
+                          "ConstantShell": 7, "EnergyRemoval": 8,
+                          "Turbulent2D": 9}[c.get("force", "None_")],
                    equilibrium=3, nx=c["nx"], ny=c["ny"], nz=c.get("nz", 1), tau=c["tau"],
                    amplitude=c.get("amplitude", (0, 0, 0)),
                    wavelength=c.get("wavelength", (1, 1, 1)))
@@ -264,18 +279,33 @@ def cpu_baseline(wl, max_seconds=15.0):
     else:
         u = o.taylor_green(cfg, spec[1])
     f = o.init_equilibrium(cfg, np.ones(cfg.cells), u)
-    del u
+    if cfg.force >= o.FORCE_CONSTANT_SHELL:
+        # spectral force: numpy transforms of the field before every step
+        spec = dict(kmin=c["force_kmin"], kmax=c["force_kmax"],
+                    removal_amplitude=c["removal_amplitude"], removal_kmin=c["removal_kmin"],
+                    removal_kmax=c["removal_kmax"])
+        every = wl.get("store_every", 1)
+        ones = np.ones(cfg.cells)
+
+        def steps(k):
+            o.run_field_force(cfg, f, k, ones, u, spec, store_every=every, threads=threads)
+    else:
+        def steps(k):
+            o.run(cfg, f, k, threads=threads)
     t0 = time.perf_counter()
-    o.run(cfg, f, 1, threads=threads)
+    steps(1)
     t1 = time.perf_counter() - t0
     n = int(max(1, min(50, (max_seconds - t1) / max(t1, 1e-6))))
     t0 = time.perf_counter()
-    o.run(cfg, f, n, threads=threads)
+    steps(n)
     t = time.perf_counter() - t0
+    del u
     return {"value": round(cfg.cells * n / t / 1e6, 3), "unit": "MLUPS", "cores": threads,
             "kind": "port",
             "sample": f"FP64 oracle (OpenMP over x, {threads} threads), {n} steps of the "
-                      f"same {cfg.nx}x{cfg.ny}x{cfg.nz} workload after 1 warm-up step"}
+                      f"same {cfg.nx}x{cfg.ny}x{cfg.nz} workload after 1 warm-up step"
+                      + (" (force field by numpy FFT before every step)"
+                         if cfg.force >= o.FORCE_CONSTANT_SHELL else "")}
 
 
 def accuracy_study(nsteps: int, every: int, n: int) -> dict:
@@ -533,7 +563,13 @@ def main(argv=None) -> int:
     hip.sync()
     t0 = time.perf_counter()
     hip.record(e0, s)
-    a.iterate(K)
+    every = wl.get("store_every")
+    if every:
+        # storing steps feed the time-dependent spectral forces (Routine.h:121-136)
+        for k in range(0, K, every):
+            a.iterate(min(every, K - k), store=True)
+    else:
+        a.iterate(K)
     hip.record(e1, s)
     a.synchronize()
     hip.sync()

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 /* Error codes. */
 enum {
@@ -54,7 +54,9 @@ enum { LBM_BGK = 1, LBM_ELBM = 2 };                                /* CollisionT
 enum { LBM_FS_NONE = 1, LBM_FS_GUO = 2, LBM_FS_SHANCHEN = 3,
        LBM_FS_EDM = 4 };                                           /* ForcingSchemeType */
 enum { LBM_FORCE_NONE = 0, LBM_FORCE_CONSTANT = 4,
-       LBM_FORCE_SINUSOIDAL = 5, LBM_FORCE_KOLMOGOROV = 6 };       /* ForceType */
+       LBM_FORCE_SINUSOIDAL = 5, LBM_FORCE_KOLMOGOROV = 6,
+       LBM_FORCE_CONSTANT_SHELL = 7, LBM_FORCE_ENERGY_REMOVAL = 8,
+       LBM_FORCE_TURBULENT_2D = 9 };                               /* ForceType */
 enum { LBM_EQ_EXACT = 1, LBM_EQ_MA3 = 3 };                         /* EquilibriumType */
 
 /* Arithmetic types (storage of populations, compute of the step). */
@@ -103,6 +105,17 @@ typedef struct {
   double residual_tol;  /* absolute H-residual tolerance |G(alpha)| of the
                            LEARNED and WARM_RESIDUAL solvers */
   int kernel;           /* LBM_KERNEL_* */
+  /* Spectral forces (Force.h:294-623), a force field recomputed by the
+   * library with hipFFT; shells are integer |k|^2 in [kmin^2, kmax^2]:
+   *  CONSTANT_SHELL: F = curl psi, psi_hat = amplitude on the shell
+   *    (MakeIncompressible, Transformer.h:301-389), set once;
+   *  ENERGY_REMOVAL: F_d = -removal_amplitude_d (rho u_d) restricted to the
+   *    removal shell, from the fields of the last storing step (the initial
+   *    fields before any), recomputed before every step;
+   *  TURBULENT_2D: the sum of both (injection: amplitude, force_kmin..kmax). */
+  int force_kmin, force_kmax;     /* forcekMin, forcekMax (Input_prod_2d.in:51-52) */
+  double removal_amplitude[3];    /* removalForceAmplitude (Input_prod_2d.in:54) */
+  int removal_kmin, removal_kmax; /* removalForcekMin/kMax (Input_prod_2d.in:56-57) */
 } lbm_config;
 
 typedef struct lbm_handle lbm_handle;
@@ -122,7 +135,8 @@ typedef struct {
 int lbm_abi_version(void);
 
 /* Fills a config with the reference defaults: D2Q9 BGK, Guo, no force,
- * Ma3, 16x16, tau 0.6, F64 storage and compute, unshifted. */
+ * Ma3, 16x16, tau 0.6, F64 storage and compute, unshifted; spectral-force
+ * shells 1..2 and removal amplitude 1e-6 (Input_prod.in:52-58). */
 int lbm_config_default(lbm_config *cfg);
 
 /* Creates a handle on the current HIP device: validates cfg, allocates the

@@ -58,6 +58,9 @@ class ForceType(enum.IntEnum):
     Constant = 4
     Sinusoidal = 5
     Kolmogorov = 6
+    ConstantShell = 7    # spectral force field, set once (Force.h:296-395)
+    EnergyRemoval = 8    # -A (rho u) on a shell, recomputed each step (Force.h:398-541)
+    Turbulent2D = 9      # ConstantShell + EnergyRemoval (Force.h:544-600)
 
 
 class EquilibriumType(enum.IntEnum):
@@ -113,13 +116,22 @@ class Config:
     alpha_mode: int = AlphaMode.NEWTON
     residual_tol: float = 1e-10
     kernel: int = KernelShape.AUTO
+    # spectral forces (Input_prod_2d.in:49-57 names)
+    force_kmin: int = 1
+    force_kmax: int = 2
+    removal_amplitude: tuple = (1e-6, 1e-6, 1e-6)
+    removal_kmin: int = 1
+    removal_kmax: int = 2
 
     def to_c(self) -> LbmConfigC:
         c = LbmConfigC()
         for name in ("lattice", "collision", "forcing_scheme", "force",
                      "equilibrium", "nx", "ny", "nz", "storage", "compute",
-                     "alpha_mode", "kernel"):
+                     "alpha_mode", "kernel", "force_kmin", "force_kmax",
+                     "removal_kmin", "removal_kmax"):
             setattr(c, name, int(getattr(self, name)))
+        for d in range(3):
+            c.removal_amplitude[d] = float(self.removal_amplitude[d])
         c.shifted = 1 if self.shifted else 0
         c.tau = float(self.tau)
         c.residual_tol = float(self.residual_tol)

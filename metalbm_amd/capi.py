@@ -42,6 +42,9 @@ class LbmConfigC(ctypes.Structure):
         ("storage", ctypes.c_int), ("shifted", ctypes.c_int),
         ("compute", ctypes.c_int), ("alpha_mode", ctypes.c_int),
         ("residual_tol", ctypes.c_double), ("kernel", ctypes.c_int),
+        ("force_kmin", ctypes.c_int), ("force_kmax", ctypes.c_int),
+        ("removal_amplitude", ctypes.c_double * 3),
+        ("removal_kmin", ctypes.c_int), ("removal_kmax", ctypes.c_int),
     ]
 
 

@@ -81,6 +81,11 @@ struct lbm_handle {
   unsigned log_threshold = 0, log_salt = 0;
   bool log_on = false;
   void* spectral = nullptr;  // spectral diagnostics state (spectral.hip)
+  // spectral forces: the force field [D][N] in FP64 and, for FP32 compute,
+  // its FP32 copy; the step kernels read it through ftab (fax = 4)
+  double* ffield = nullptr;
+  float* ffield32 = nullptr;
+  bool ffield_built = false;
 };
 
 namespace {
@@ -118,9 +123,20 @@ int validate(const lbm_config* c, std::string& why) {
     why = "forcing_scheme must be None (1), Guo (2), ShanChen (3) or EDM (4)";
     return LBM_E_INVALID;
   }
-  if (c->force != LBM_FORCE_NONE && c->force != LBM_FORCE_CONSTANT &&
-      c->force != LBM_FORCE_SINUSOIDAL && c->force != LBM_FORCE_KOLMOGOROV) {
-    why = "force must be None (0), Constant (4), Sinusoidal (5) or Kolmogorov (6)";
+  if (c->force != LBM_FORCE_NONE && (c->force < LBM_FORCE_CONSTANT ||
+                                      c->force > LBM_FORCE_TURBULENT_2D)) {
+    why = "force must be None (0), Constant (4), Sinusoidal (5), Kolmogorov (6), "
+          "ConstantShell (7), EnergyRemoval (8) or Turbulent2D (9)";
+    return LBM_E_INVALID;
+  }
+  if ((c->force == LBM_FORCE_CONSTANT_SHELL || c->force == LBM_FORCE_TURBULENT_2D) &&
+      (c->force_kmin < 0 || c->force_kmax < c->force_kmin)) {
+    why = "spectral force shell needs 0 <= force_kmin <= force_kmax";
+    return LBM_E_INVALID;
+  }
+  if ((c->force == LBM_FORCE_ENERGY_REMOVAL || c->force == LBM_FORCE_TURBULENT_2D) &&
+      (c->removal_kmin < 0 || c->removal_kmax < c->removal_kmin)) {
+    why = "energy removal shell needs 0 <= removal_kmin <= removal_kmax";
     return LBM_E_INVALID;
   }
   if (c->equilibrium != LBM_EQ_MA3 && c->equilibrium != LBM_EQ_EXACT) {
@@ -242,6 +258,29 @@ int build_force_tables(lbm_handle* h) {
         tab[0][y] = c.amplitude[0] *
                     std::sin((double)((unsigned)y * 2u) * M_PI / c.wavelength[0]);
       break;
+    case LBM_FORCE_CONSTANT_SHELL:
+    case LBM_FORCE_ENERGY_REMOVAL:
+    case LBM_FORCE_TURBULENT_2D: {
+      // a per-cell field (Force<Generic>::setForce, Force.h:39-48), filled
+      // by the spectral-force update before the first step
+      const size_t n = (size_t)h->D * h->N;
+      HIP_TRY(h, hipMalloc(&h->ffield, n * sizeof(double)));
+      HIP_TRY(h, hipMemset(h->ffield, 0, n * sizeof(double)));
+      if (h->ctype != TYPE_F64) {
+        HIP_TRY(h, hipMalloc(&h->ffield32, n * sizeof(float)));
+        HIP_TRY(h, hipMemset(h->ffield32, 0, n * sizeof(float)));
+      }
+      for (int d = 0; d < h->D; ++d) {
+        h->fax[d] = 4;
+        h->ftab[d] = h->ctype == TYPE_F64 ? (void*)(h->ffield + (size_t)d * h->N)
+                                           : (void*)(h->ffield32 + (size_t)d * h->N);
+      }
+      // EnergyRemoval reads rho and u of the stored fields (the initial
+      // fields until a storing step): zero until an initialiser writes them
+      HIP_TRY(h, hipMalloc(&h->fields, (size_t)h->N * (2 + 2 * h->D) * sizeof(double)));
+      HIP_TRY(h, hipMemset(h->fields, 0, (size_t)h->N * (2 + 2 * h->D) * sizeof(double)));
+      return LBM_OK;
+    }
     default:
       break;
   }
@@ -287,9 +326,30 @@ int ensure_fields(lbm_handle* h) {
   return LBM_OK;
 }
 
+// The reference's field list holds the initial density and velocity until
+// the first storing step (Initialize.h:20-70, Algorithm.h:150-160); the
+// EnergyRemoval and Turbulent2D forces read them (Force.h:416-431).  drho may
+// be NULL for a uniform rho0; du is the device velocity [D][N].
+hipError_t seed_fields(lbm_handle* h, const double* drho, double rho0, const double* du) {
+  if (h->cfg.force != LBM_FORCE_ENERGY_REMOVAL && h->cfg.force != LBM_FORCE_TURBULENT_2D)
+    return hipSuccess;
+  hipError_t e = drho ? hipMemcpyAsync(h->fields, drho, (size_t)h->N * sizeof(double),
+                                       hipMemcpyDeviceToDevice, h->stream)
+                      : launch_fill(TYPE_F64, h->fields, rho0, h->N, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h->fields + h->N, du, (size_t)h->N * h->D * sizeof(double),
+                       hipMemcpyDeviceToDevice, h->stream);
+  return e;
+}
+
 void release(lbm_handle* h) {
   lbm::release_spectral(h->spectral);
   h->spectral = nullptr;
+  if (h->ffield) {  // ftab points into the force field
+    for (void*& t : h->ftab) t = nullptr;
+    (void)hipFree(h->ffield);
+    if (h->ffield32) (void)hipFree(h->ffield32);
+  }
   for (void* p : {h->f[0], h->f[1], h->alpha, (void*)h->fields,
                   (void*)h->staging, (void*)h->reduce_buf, h->ftab[0],
                   h->ftab[1], h->ftab[2], (void*)h->stats, (void*)h->mlp, h->mlp_packed,
@@ -360,6 +420,22 @@ int lbm::device_fields(lbm_handle* h, DeviceFields& out) {
 
 void*& lbm::spectral_slot(lbm_handle* h) { return h->spectral; }
 
+int lbm::force_field(lbm_handle* h, ForceField& out) {
+  if (!h->ffield) return fail(h, LBM_E_STATE, "the configured force has no force field");
+  out.f64 = h->ffield;
+  out.f32 = h->ffield32;
+  out.rho = h->fields;
+  out.u = h->fields + h->N;
+  out.D = h->D;
+  out.n[0] = h->cfg.nx;
+  out.n[1] = h->cfg.ny;
+  out.n[2] = h->cfg.nz;
+  out.N = h->N;
+  out.stream = h->stream;
+  out.cfg = h->cfg;
+  return LBM_OK;
+}
+
 extern "C" {
 
 int lbm_abi_version(void) { return LBM_ABI_VERSION; }
@@ -383,6 +459,11 @@ int lbm_config_default(lbm_config* c) {
   c->alpha_mode = LBM_ALPHA_NEWTON;
   c->residual_tol = 1e-10;
   c->kernel = LBM_KERNEL_AUTO;
+  c->force_kmin = 1;  // Input_prod.in:52-58
+  c->force_kmax = 2;
+  for (int d = 0; d < 3; ++d) c->removal_amplitude[d] = 1e-6;
+  c->removal_kmin = 1;
+  c->removal_kmax = 2;
   return LBM_OK;
 }
 
@@ -533,6 +614,7 @@ int lbm_init_equilibrium(lbm_handle* h, const double* rho, const double* u) {
   HIP_TRY(h, launch_init_equilibrium(h->cfg.lattice, h->cfg.equilibrium,
                                      h->cfg.storage, h->cfg.shifted,
                                      h->f[h->cur], drho, 0.0, du, h->N, h->stream));
+  HIP_TRY(h, seed_fields(h, drho, 0.0, du));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return LBM_OK;
 }
@@ -546,6 +628,7 @@ int lbm_init_noise(lbm_handle* h, double rho0, double amp, uint64_t seed) {
                                      h->cfg.storage, h->cfg.shifted,
                                      h->f[h->cur], nullptr, rho0, du, h->N,
                                      h->stream));
+  HIP_TRY(h, seed_fields(h, nullptr, rho0, du));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return LBM_OK;
 }
@@ -576,6 +659,7 @@ int lbm_init_taylor_green(lbm_handle* h, double rho0, double u0) {
   if (e == hipSuccess)
     e = launch_init_equilibrium(c.lattice, c.equilibrium, c.storage, c.shifted,
                                 h->f[h->cur], nullptr, rho0, du, h->N, h->stream);
+  if (e == hipSuccess) e = seed_fields(h, nullptr, rho0, du);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(tab);
   if (e != hipSuccess) return hip_fail(h, e, "Taylor-Green initialisation");
@@ -731,6 +815,7 @@ int lbm_init_spectral(lbm_handle* h, double rho0, double urms, int kmax,
   if (e == hipSuccess)
     e = launch_init_equilibrium(c.lattice, c.equilibrium, c.storage, c.shifted,
                                 h->f[h->cur], nullptr, rho0, du, h->N, h->stream);
+  if (e == hipSuccess) e = seed_fields(h, nullptr, rho0, du);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(buf);
   if (e != hipSuccess) return hip_fail(h, e, "spectral initialisation");
@@ -810,7 +895,16 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
   // Vector kernel: one wavefront per 64 V-cell segment, 4 per workgroup.
   const int grid = h->vec ? (h->rows * h->tiles_per_row + 3) / 4
                           : h->rows * h->tiles_per_row;
+  const bool field_force = h->ffield != nullptr;
+  const bool time_dependent = h->cfg.force == LBM_FORCE_ENERGY_REMOVAL ||
+                              h->cfg.force == LBM_FORCE_TURBULENT_2D;
   for (int s = 0; s < nsteps; ++s) {
+    // Collision::update before the step (Algorithm.h:338, Collision.h:96-99);
+    // ConstantShell is set once (Collision.h:51-54)
+    if (field_force && (time_dependent || !h->ffield_built)) {
+      if (int rc = lbm::update_spectral_force(h)) return rc;
+      h->ffield_built = true;
+    }
     a.src = h->f[h->cur];
     a.dst = h->f[1 - h->cur];
     a.store = (store_last && s == nsteps - 1) ? 1 : 0;

@@ -33,4 +33,20 @@ int device_fields(lbm_handle* h, DeviceFields& out);
 void*& spectral_slot(lbm_handle* h);
 void release_spectral(void* state);
 
+// The force field of the spectral forces and what its update reads.
+struct ForceField {
+  double* f64;          // [D][N] force field, FP64
+  float* f32;           // its FP32 copy for FP32 compute, else NULL
+  const double* rho;    // [N] density of the last storing step / initialiser
+  const double* u;      // [D][N] velocity of the same
+  int D, n[3];
+  int64_t N;
+  hipStream_t stream;
+  lbm_config cfg;
+};
+int force_field(lbm_handle* h, ForceField& out);
+// Recomputes the force field (ConstantShell, EnergyRemoval, Turbulent2D);
+// defined in spectral.hip.
+int update_spectral_force(lbm_handle* h);
+
 }  // namespace lbm

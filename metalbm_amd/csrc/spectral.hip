@@ -59,12 +59,15 @@ struct State {
   double* partial = nullptr;  // kReduceBlocks + 1
   double* bins = nullptr;     // shell partials: shell_blocks x min(nshells, window)
   int shell_blocks = 0;
+  hipfftHandle inv_d = 0;     // inverse transform of D components (force field)
+  bool have_inv_d = false;
 };
 
 void free_state(State* s) {
   if (!s) return;
   if (s->have_fwd) hipfftDestroy(s->fwd);
   if (s->have_inv) hipfftDestroy(s->inv);
+  if (s->have_inv_d) hipfftDestroy(s->inv_d);
   for (void* p : {(void*)s->real, (void*)s->hat, (void*)s->what, (void*)s->omega,
                   (void*)s->spec, (void*)s->partial, (void*)s->bins})
     if (p) (void)hipFree(p);
@@ -273,6 +276,70 @@ __global__ void k_shell_final(const double* __restrict__ partial, int blocks, in
   if (threadIdx.x == 0) out[b] = t * scale;
 }
 
+// out[d][x] = rho[x] u[d][x]: the momentum EnergyRemoval transforms
+// (Force.h:416-431).
+__global__ void k_momentum(const double* __restrict__ rho, const double* __restrict__ u, int D,
+                           int64_t N, double* __restrict__ out) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const double r = rho[x];
+  for (int d = 0; d < D; ++d) out[d * N + x] = r * u[d * N + x];
+}
+
+struct ForceSpec {
+  int injection, kmin2, kmax2;  // ConstantShell shell (integer |k|^2 bounds)
+  double A[3];                  // its psi_hat amplitude per component
+  int removal, rmin2, rmax2;    // EnergyRemoval shell
+  double R[3];                  // its amplitude per component
+  double scale;                 // the backward transform's 1/N
+};
+
+// Fourier-space force, in place on hat (read as the momentum spectrum when
+// removal is on): EnergyRemoval -R_d p_hat_d on its shell (Force.h:436-497),
+// plus ConstantShell's MakeIncompressible of psi_hat = A on its shell
+// (Force.h:327-393; Transformer.h:320-334 in 2D: F = (i ky psi, -i kx psi);
+// in 3D the Curl of Transformer.h:216-275: F = i k x psi), times 1/N.
+__global__ void k_force_hat(double2* __restrict__ hat, Dims g, ForceSpec f) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= g.M) return;
+  const ModeIndex ix = mode_index(m, g.D, g.n);
+  long long k2 = 0;
+  for (int d = 0; d < g.D; ++d) {
+    const long long k = wavenumber(ix.i[d], g.n[d]);
+    k2 += k * k;
+  }
+  double2 F[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  if (f.removal && k2 >= f.rmin2 && k2 <= f.rmax2)
+    for (int d = 0; d < g.D; ++d) {
+      const double2 p = hat[(int64_t)d * g.M + m];
+      F[d] = make_double2(-f.R[d] * p.x, -f.R[d] * p.y);
+    }
+  if (f.injection && k2 >= f.kmin2 && k2 <= f.kmax2) {
+    const int nl = g.n[g.D - 1], l = ix.i[g.D - 1];
+    const bool self_paired = l == 0 || ((nl & 1) == 0 && l == nl / 2);
+    double k[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < g.D; ++d) k[d] = dfactor(ix.i[d], g.n[d], false, self_paired);
+    double c[3];  // psi_hat is real, so i (k x psi) is imaginary: F += i c
+    if (g.D == 2) {
+      c[0] = k[1] * f.A[0];
+      c[1] = -k[0] * f.A[0];
+      c[2] = 0.0;
+    } else {
+      c[0] = k[1] * f.A[2] - k[2] * f.A[1];
+      c[1] = k[2] * f.A[0] - k[0] * f.A[2];
+      c[2] = k[0] * f.A[1] - k[1] * f.A[0];
+    }
+    for (int d = 0; d < g.D; ++d) F[d].y += c[d];
+  }
+  for (int d = 0; d < g.D; ++d)
+    hat[(int64_t)d * g.M + m] = make_double2(F[d].x * f.scale, F[d].y * f.scale);
+}
+
+__global__ void k_to_f32(const double* __restrict__ src, int64_t n, float* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
 unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 Dims dims_of(const State& s) {
@@ -358,6 +425,57 @@ int check_scaling(lbm_handle* h, int scaling) {
 }  // namespace
 
 void lbm::release_spectral(void* state) { free_state(static_cast<State*>(state)); }
+
+int lbm::update_spectral_force(lbm_handle* h) {
+  lbm::ForceField ff;
+  if (int rc = lbm::force_field(h, ff)) return rc;
+  lbm::DeviceFields f{ff.rho, ff.u, nullptr, ff.D, {ff.n[0], ff.n[1], ff.n[2]}, ff.N, ff.stream};
+  State* s = nullptr;
+  if (int rc = state_of(h, f, s)) return rc;
+  const hipStream_t st = ff.stream;
+  if (!s->have_inv_d) {
+    int dims[3] = {s->n[0], s->n[1], s->n[2]};
+    const hipfftResult r = hipfftPlanMany(&s->inv_d, s->D, dims, nullptr, 1, (int)s->M, nullptr,
+                                          1, (int)s->N, HIPFFT_Z2D, s->D);
+    if (r != HIPFFT_SUCCESS) return fft_fail(h, r, "force transform plan");
+    s->have_inv_d = true;
+  }
+  const lbm_config& c = ff.cfg;
+  ForceSpec spec{};
+  spec.injection = c.force == LBM_FORCE_CONSTANT_SHELL || c.force == LBM_FORCE_TURBULENT_2D;
+  spec.removal = c.force == LBM_FORCE_ENERGY_REMOVAL || c.force == LBM_FORCE_TURBULENT_2D;
+  spec.kmin2 = c.force_kmin * c.force_kmin;
+  spec.kmax2 = c.force_kmax * c.force_kmax;
+  spec.rmin2 = c.removal_kmin * c.removal_kmin;
+  spec.rmax2 = c.removal_kmax * c.removal_kmax;
+  for (int d = 0; d < 3; ++d) {
+    spec.A[d] = c.amplitude[d];
+    spec.R[d] = c.removal_amplitude[d];
+  }
+  spec.scale = 1.0 / (double)s->N;
+  if (spec.removal) {
+    hipLaunchKernelGGL(k_momentum, dim3(grid_of(s->N)), dim3(kBlock), 0, st, ff.rho, ff.u,
+                       s->D, s->N, s->real);
+    if (int rc = lbm::record_hip(h, hipGetLastError(), "momentum kernel")) return rc;
+    hipfftResult r = hipfftSetStream(s->fwd, st);
+    if (r == HIPFFT_SUCCESS)
+      r = hipfftExecD2Z(s->fwd, (hipfftDoubleReal*)s->real, (hipfftDoubleComplex*)s->hat);
+    if (r != HIPFFT_SUCCESS) return fft_fail(h, r, "momentum transform");
+  }
+  hipLaunchKernelGGL(k_force_hat, dim3(grid_of(s->M)), dim3(kBlock), 0, st, s->hat,
+                     dims_of(*s), spec);
+  if (int rc = lbm::record_hip(h, hipGetLastError(), "force spectrum kernel")) return rc;
+  hipfftResult r = hipfftSetStream(s->inv_d, st);
+  if (r == HIPFFT_SUCCESS)
+    r = hipfftExecZ2D(s->inv_d, (hipfftDoubleComplex*)s->hat, (hipfftDoubleReal*)ff.f64);
+  if (r != HIPFFT_SUCCESS) return fft_fail(h, r, "force transform");
+  if (ff.f32) {
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_of(s->D * s->N)), dim3(kBlock), 0, st, ff.f64,
+                       (int64_t)s->D * s->N, ff.f32);
+    if (int rc = lbm::record_hip(h, hipGetLastError(), "force conversion")) return rc;
+  }
+  return LBM_OK;
+}
 
 extern "C" {
 

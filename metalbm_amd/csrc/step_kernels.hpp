@@ -40,7 +40,7 @@ struct StepArgs {
   double beta;          // 1 / (2 tau)           (Collision.h:122)
   double guo_coef;      // 1 - 1 / (2 tau0)      (ForcingScheme.h:115)
   const void* ftab[3];  // per-component force tables (compute type) or NULL
-  int fax[3];           // axis indexing table d: 0 x, 1 y, 2 z, 3 constant
+  int fax[3];           // axis indexing table d: 0 x, 1 y, 2 z, 3 constant, 4 cell
   int store;            // write output fields on this step
   double* rho_out;
   double* uh_out;       // [D][N]
@@ -150,11 +150,13 @@ __device__ __forceinline__ int self_index(const StepArgs& a, const CellPos& c) {
   else return c.x * a.ny + c.y;
 }
 
-// Force_::setForce from the host-built tables (Force.h:104-292).
+// Force_::setForce from the host-built tables (Force.h:104-292) or, for the
+// spectral forces, from the per-cell force field (fax 4: Force<Generic>::
+// setForce, Force.h:39-48).
 template <class L, class C>
 __device__ __forceinline__ void force_of_cell(const StepArgs& a,
                                               const CellPos& c, C* F) {
-  const int pos[4] = {c.x, c.y, c.z, 0};
+  const int pos[5] = {c.x, c.y, c.z, 0, self_index<L>(a, c)};
   F[0] = F[1] = F[2] = C(0);
 #pragma unroll
   for (int d = 0; d < L::dimD; ++d) {

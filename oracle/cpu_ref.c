@@ -119,7 +119,9 @@ static int check_config(const ref_config *cfg) {
   if (cfg->forcing_scheme < REF_FS_NONE || cfg->forcing_scheme > REF_FS_EDM)
     return -4;
   if (cfg->force != REF_FORCE_NONE && cfg->force != REF_FORCE_CONSTANT &&
-      cfg->force != REF_FORCE_SINUSOIDAL && cfg->force != REF_FORCE_KOLMOGOROV)
+      cfg->force != REF_FORCE_SINUSOIDAL && cfg->force != REF_FORCE_KOLMOGOROV &&
+      cfg->force != REF_FORCE_CONSTANT_SHELL && cfg->force != REF_FORCE_ENERGY_REMOVAL &&
+      cfg->force != REF_FORCE_TURBULENT_2D)
     return -5;
   /* There is no Exact equilibrium for D3Q19 (Equilibrium.h:36-126). */
   if (cfg->equilibrium == REF_EQ_EXACT && cfg->lattice == REF_LATTICE_D3Q19)
@@ -348,6 +350,7 @@ typedef struct {
   double omega_eq;  /* 2 beta                (Collision.h:147) */
   double beta;      /* 1 / (2 tau)           (Collision.h:122) */
   double guo_coef;  /* 1 - 1 / (2 tau0)      (ForcingScheme.h:115) */
+  const double *ffield;  /* force field [D][N] (Generic setForce) or NULL */
 } coefs_t;
 
 static inline int wrap(int v, int n) {
@@ -383,7 +386,13 @@ static inline void cell_step(const lat_t *L, const ref_config *cfg,
   for (int d = 1; d < D; ++d) u2 += u[d] * u[d];
   /* setForce (Collision.h:81-88). */
   double F[3];
-  ref_force_at(cfg, x, y, z, F);
+  if (k->ffield) {
+    /* Force<Generic>::setForce reads the force field (Force.h:39-48). */
+    F[0] = F[1] = F[2] = 0.0;
+    for (int d = 0; d < D; ++d) F[d] = k->ffield[(int64_t)d * N + cell];
+  } else {
+    ref_force_at(cfg, x, y, z, F);
+  }
   const int eq = cfg->equilibrium, scheme = cfg->forcing_scheme;
   double a = 2.0;
   if (cfg->collision == REF_COLLISION_BGK) {
@@ -431,9 +440,10 @@ static inline void cell_step(const lat_t *L, const ref_config *cfg,
 
 static void step_lattice(const lat_t *L, const ref_config *cfg,
                          const double *prev, double *next, double *alpha,
-                         double *rho, double *uh, double *F_out,
-                         ref_stats *stats) {
+                         const double *ffield, double *rho, double *uh,
+                         double *F_out, ref_stats *stats) {
   coefs_t k;
+  k.ffield = ffield;
   k.beta = 1.0 / (2.0 * cfg->tau);
   k.omega_f = 1. - 2. * k.beta;
   k.omega_eq = 2. * k.beta;
@@ -468,21 +478,30 @@ static void step_lattice(const lat_t *L, const ref_config *cfg,
 int ref_step(const ref_config *cfg, const double *prev, double *next,
              double *alpha, double *rho, double *u_hydro, double *force_out,
              ref_stats *stats) {
+  return ref_step_field(cfg, prev, next, alpha, NULL, rho, u_hydro, force_out,
+                        stats);
+}
+
+int ref_step_field(const ref_config *cfg, const double *prev, double *next,
+                   double *alpha, const double *force_field, double *rho,
+                   double *u_hydro, double *force_out, ref_stats *stats) {
   int e = check_config(cfg);
   if (e) return e;
   if (cfg->collision == REF_COLLISION_ELBM && !alpha) return -9;
+  /* The spectral forces live in a force field (Force.h:294-623). */
+  if (cfg->force >= REF_FORCE_CONSTANT_SHELL && !force_field) return -10;
   /* Dispatch on a constant table so the compiler specialises the loops. */
   switch (cfg->lattice) {
     case REF_LATTICE_D2Q9:
-      step_lattice(&LAT_D2Q9, cfg, prev, next, alpha, rho, u_hydro, force_out,
-                   stats);
+      step_lattice(&LAT_D2Q9, cfg, prev, next, alpha, force_field, rho, u_hydro,
+                   force_out, stats);
       break;
     case REF_LATTICE_D3Q19:
-      step_lattice(&LAT_D3Q19, cfg, prev, next, alpha, rho, u_hydro,
+      step_lattice(&LAT_D3Q19, cfg, prev, next, alpha, force_field, rho, u_hydro,
                    force_out, stats);
       break;
     default:
-      step_lattice(&LAT_D3Q27, cfg, prev, next, alpha, rho, u_hydro,
+      step_lattice(&LAT_D3Q27, cfg, prev, next, alpha, force_field, rho, u_hydro,
                    force_out, stats);
       break;
   }

@@ -50,7 +50,10 @@ enum {
   REF_FORCE_NONE = 0,
   REF_FORCE_CONSTANT = 4,
   REF_FORCE_SINUSOIDAL = 5,
-  REF_FORCE_KOLMOGOROV = 6
+  REF_FORCE_KOLMOGOROV = 6,
+  REF_FORCE_CONSTANT_SHELL = 7,   /* field forces: supplied to ref_step_field */
+  REF_FORCE_ENERGY_REMOVAL = 8,
+  REF_FORCE_TURBULENT_2D = 9
 };
 enum { REF_EQ_EXACT = 1, REF_EQ_MA3 = 3 };
 
@@ -105,6 +108,15 @@ void ref_init_equilibrium(const ref_config *cfg, const double *rho,
 int ref_step(const ref_config *cfg, const double *prev, double *next,
              double *alpha, double *rho, double *u_hydro, double *force_out,
              ref_stats *stats);
+
+/* ref_step with the force read per cell from force_field[D][N] (Generic
+ * setForce, Force.h:39-48) instead of the analytic forces; required for the
+ * spectral forces (ConstantShell, EnergyRemoval, Turbulent2D), whose field
+ * the caller computes (oracle/spectral.py).  force_field may be NULL for the
+ * analytic forces. */
+int ref_step_field(const ref_config *cfg, const double *prev, double *next,
+                   double *alpha, const double *force_field, double *rho,
+                   double *u_hydro, double *force_out, ref_stats *stats);
 
 /* Run nsteps with a two-lattice ping-pong; the result ends in f.
  * scratch must hold Q*N doubles.  Fields (may be NULL) are those of the last

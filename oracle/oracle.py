@@ -24,6 +24,7 @@ D2Q9, D3Q19, D3Q27 = 3, 9, 10
 BGK, ELBM = 1, 2
 FS_NONE, FS_GUO, FS_SHANCHEN, FS_EDM = 1, 2, 3, 4
 FORCE_NONE, FORCE_CONSTANT, FORCE_SINUSOIDAL, FORCE_KOLMOGOROV = 0, 4, 5, 6
+FORCE_CONSTANT_SHELL, FORCE_ENERGY_REMOVAL, FORCE_TURBULENT_2D = 7, 8, 9
 EQ_EXACT, EQ_MA3 = 1, 3
 ALPHA_SMALL, ALPHA_CAPPED, ALPHA_NEWTON, ALPHA_FALLBACK = 0, 1, 2, 3
 
@@ -70,6 +71,7 @@ def lib():
         L.ref_force_at.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.ref_init_equilibrium.argtypes = [cp, dp, dp, dp]
         L.ref_step.argtypes = [cp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(_Stats)]
+        L.ref_step_field.argtypes = [cp, dp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(_Stats)]
         L.ref_run.argtypes = [cp, dp, dp, dp, ctypes.c_int, dp, dp, dp,
                               ctypes.POINTER(_Stats)]
         L.ref_elbm_alpha.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double,
@@ -245,6 +247,73 @@ def run(cfg: Config, f0: np.ndarray, nsteps: int, alpha0: np.ndarray | None = No
         raise ValueError(f"oracle rejected the configuration (code {rc})")
     return RunResult(f, alpha, rho, u, F, np.array(st.branch[:], dtype=np.int64),
                      int(st.newton_iterations))
+
+
+def step_field(cfg: Config, f: np.ndarray, alpha: np.ndarray | None, force_field: np.ndarray,
+               threads: int = 1):
+    """One step with the force read from force_field[D, N] (the Generic
+    setForce, Force.h:39-48).  Returns (f_next, rho, u_hydro, force); alpha
+    (ELBM warm start) is updated in place."""
+    N, Q, D = cfg.cells, cfg.q, cfg.d
+    prev = np.ascontiguousarray(f, dtype=np.float64).reshape(Q, N)
+    nxt = np.empty_like(prev)
+    ff = np.ascontiguousarray(force_field, dtype=np.float64).reshape(D, N)
+    if alpha is None:
+        alpha = np.full(N, 2.0)
+    rho, u, F = np.zeros(N), np.zeros((D, N)), np.zeros((D, N))
+    L = lib()
+    L.ref_set_threads(threads)
+    k = cfg.c()
+    rc = L.ref_step_field(ctypes.byref(k), _p(prev), _p(nxt), _p(alpha), _p(ff), _p(rho),
+                          _p(u), _p(F), None)
+    if rc != 0:
+        raise ValueError(f"oracle rejected the configuration (code {rc})")
+    return nxt, rho, u, F
+
+
+def spectral_force_field(cfg: Config, spec: dict, rho: np.ndarray, u: np.ndarray) -> np.ndarray:
+    """The force field of a spectral force (oracle/spectral.py) from the
+    stored fields rho[N], u[D, N].  spec: kmin, kmax, removal_amplitude,
+    removal_kmin, removal_kmax (the input file's forcekMin ... names)."""
+    from oracle import spectral as sp
+    shape = (cfg.nx, cfg.ny) if cfg.d == 2 else (cfg.nx, cfg.ny, cfg.nz)
+    if cfg.force == FORCE_CONSTANT_SHELL:
+        return sp.constant_shell_force(shape, cfg.amplitude, spec["kmin"], spec["kmax"])
+    if cfg.force == FORCE_ENERGY_REMOVAL:
+        return sp.energy_removal_force(rho, u, shape, spec["removal_amplitude"],
+                                       spec["removal_kmin"], spec["removal_kmax"])
+    if cfg.force == FORCE_TURBULENT_2D:
+        return sp.turbulent_force(rho, u, shape, cfg.amplitude, spec["kmin"], spec["kmax"],
+                                  spec["removal_amplitude"], spec["removal_kmin"],
+                                  spec["removal_kmax"])
+    raise ValueError("not a spectral force")
+
+
+def run_field_force(cfg: Config, f0: np.ndarray, nsteps: int, rho0: np.ndarray,
+                    u0: np.ndarray, spec: dict, store_every: int = 1,
+                    alpha0: np.ndarray | None = None, threads: int = 1):
+    """Steps with a spectral force as Algorithm::iterate runs them: the force
+    field is recomputed before every step (Collision::update, Algorithm.h:338;
+    ConstantShell once) from the fields of the last storing step, which are
+    the initial fields (rho0, u0) until the first (Initialize.h:20-70); every
+    store_every-th step stores.  Returns (f, alpha, rho, u, F) of the last
+    step."""
+    N, Q, D = cfg.cells, cfg.q, cfg.d
+    f = np.array(f0, dtype=np.float64).reshape(Q, N)
+    alpha = (np.full(N, 2.0) if alpha0 is None
+             else np.array(alpha0, dtype=np.float64).reshape(N).copy())
+    rho_s = np.array(rho0, dtype=np.float64).reshape(N)
+    u_s = np.array(u0, dtype=np.float64).reshape(D, N)
+    F = None
+    out = None
+    for step in range(1, nsteps + 1):
+        if F is None or cfg.force != FORCE_CONSTANT_SHELL:
+            F = spectral_force_field(cfg, spec, rho_s, u_s)
+        f, rho, u, Fo = step_field(cfg, f, alpha, F, threads)
+        if step % store_every == 0:
+            rho_s, u_s = rho, u
+        out = (rho, u, Fo)
+    return f, alpha, out[0], out[1], out[2]
 
 
 def elbm_alpha(lattice: int, f, fneq, alpha_guess: float = 2.0):

@@ -122,3 +122,66 @@ def spectrum(v, shape, scaling=REFERENCE, forcing=False):
     if phys:
         return out / (N * float(N))
     return out if forcing else out / N
+
+
+# -- spectral forces (Force.h:294-623) ----------------------------------------
+# Shells are integer |k|^2 in [kmin^2, kmax^2] (Force.h:372, 463 kNormSquared).
+# The reference's 3D mirror writes on the k_z = 0 plane use the y index
+# n_y - 0 = n_y for k_y = 0 (Force.h:339-352), one x row past the intended
+# mirror (and past the array for k_x = 1); restated here is the intended
+# shell -- every half-spectrum mode of the shell -- which is what the 2D
+# code (where that index is overwritten with 0) computes.
+
+def _shell(shape, kmin, kmax):
+    k, _ = _axes_k(shape)
+    k2 = sum(kk * kk for kk in k)
+    return (k2 >= kmin * kmin) & (k2 <= kmax * kmax)
+
+
+def constant_shell_force(shape, amplitude, kmin, kmax):
+    """ConstantShell::initTempArray + setForceArray (Force.h:327-393, 318-325):
+    psi_hat_c = amplitude[c] on the shell, F = MakeIncompressible(psi)
+    (Transformer.h:320-334 in 2D: F_hat = (i ky psi_hat, -i kx psi_hat); 3D:
+    the Curl, Transformer.h:216-275) with integer wavenumbers, and the
+    backward transform's 1/N.  Returns F[D, N]."""
+    shape = tuple(int(s) for s in shape)
+    D = len(shape)
+    k = _derivative_k(shape, physical=False)
+    mask = _shell(shape, kmin, kmax).astype(np.float64)
+    if D == 2:
+        psi = amplitude[0] * mask
+        Fh = [1j * k[1] * psi, -1j * k[0] * psi]
+    else:
+        psi = [amplitude[c] * mask for c in range(3)]
+        Fh = [1j * (k[1] * psi[2] - k[2] * psi[1]),
+              1j * (k[2] * psi[0] - k[0] * psi[2]),
+              1j * (k[0] * psi[1] - k[1] * psi[0])]
+    ax = tuple(range(D))
+    return np.stack([np.fft.irfftn(h, s=shape, axes=ax).reshape(-1) for h in Fh])
+
+
+def energy_removal_force(rho, u, shape, amplitude, kmin, kmax):
+    """EnergyRemoval::setForceArray (Force.h:416-497): F_d = -amplitude_d
+    (rho u_d) restricted to the shell, through the forward transform, the
+    shell product and the backward transform with its 1/N.  rho[N], u[D, N]
+    are the stored fields.  Returns F[D, N]."""
+    shape = tuple(int(s) for s in shape)
+    D = len(shape)
+    mask = _shell(shape, kmin, kmax)
+    rho = np.asarray(rho, dtype=np.float64).reshape(-1)
+    ax = tuple(range(D))
+    out = []
+    for d in range(D):
+        ph = np.fft.rfftn((rho * np.asarray(u[d], dtype=np.float64).reshape(-1)).reshape(shape),
+                          axes=ax)
+        out.append(np.fft.irfftn(np.where(mask, -amplitude[d] * ph, 0.0), s=shape,
+                                 axes=ax).reshape(-1))
+    return np.stack(out)
+
+
+def turbulent_force(rho, u, shape, amplitude, kmin, kmax, removal_amplitude, removal_kmin,
+                    removal_kmax):
+    """Turbulent2D::setForceArray (Force.h:565-588): injection (ConstantShell)
+    plus removal (EnergyRemoval), summed in real space."""
+    return (constant_shell_force(shape, amplitude, kmin, kmax)
+            + energy_removal_force(rho, u, shape, removal_amplitude, removal_kmin, removal_kmax))

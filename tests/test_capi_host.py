@@ -35,12 +35,14 @@ def test_every_declared_symbol_is_exported(capi):
 
 def test_abi_version_and_defaults(capi):
     L = capi.lib()
-    assert L.lbm_abi_version() == 1
+    assert L.lbm_abi_version() == 2
     c = capi.LbmConfigC()
     assert L.lbm_config_default(ctypes.byref(c)) == 0
     assert (c.lattice, c.collision, c.forcing_scheme, c.equilibrium) == (3, 1, 2, 3)
     assert (c.nx, c.ny, c.nz, c.storage, c.compute, c.shifted) == (16, 16, 1, 0, 0, 0)
     assert c.tau == 0.6 and c.residual_tol == 1e-10
+    assert (c.force_kmin, c.force_kmax, c.removal_kmin, c.removal_kmax) == (1, 2, 1, 2)
+    assert list(c.removal_amplitude) == [1e-6] * 3
 
 
 BAD = [
@@ -48,6 +50,9 @@ BAD = [
     (dict(collision=7), "collision"),
     (dict(forcing_scheme=0), "forcing_scheme"),
     (dict(force=1), "force"),
+    (dict(force=10), "force"),
+    (dict(force=7, force_kmin=3, force_kmax=2), "force_kmin"),
+    (dict(force=9, removal_kmin=-1), "removal_kmin"),
     (dict(lattice=9, nz=4, equilibrium=1), "Exact"),
     (dict(nz=2), "nz must be 1"),
     (dict(nx=0), "positive"),

@@ -1,0 +1,135 @@
+"""Spectral forces on the GPU (ConstantShell, EnergyRemoval, Turbulent2D,
+Force.h:294-623): the force field against the numpy oracle, one step against
+the C oracle fed the same field (bitwise for FP64 BGK), and whole runs with
+the reference's update order (field recomputed before every step from the
+fields of the last storing step) against the oracle's own fields.  The two
+FFT implementations round differently, so field and run comparisons are
+relative to the largest magnitude."""
+import numpy as np
+import pytest
+
+from metalbm_amd import (Algorithm, CollisionType, Config, ForceType, ForcingSchemeType,
+                         LatticeType, Precision)
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+SPEC = dict(force_kmin=2, force_kmax=4, removal_amplitude=(1e-2, 2e-2, 1e-2),
+            removal_kmin=1, removal_kmax=3)
+CASES = [
+    ("d2q9-turbulent2d-edm", Config(lattice=LatticeType.D2Q9, nx=32, ny=24,
+                                    forcing_scheme=ForcingSchemeType.ExactDifferenceMethod,
+                                    force=ForceType.Turbulent2D, amplitude=(5e-3, 0, 0),
+                                    tau=0.6, **SPEC)),
+    ("d2q9-elbm-turbulent2d", Config(lattice=LatticeType.D2Q9, nx=24, ny=20,
+                                     collision=CollisionType.ELBM,
+                                     forcing_scheme=ForcingSchemeType.ExactDifferenceMethod,
+                                     force=ForceType.Turbulent2D, amplitude=(5e-3, 0, 0),
+                                     tau=0.5003, **SPEC)),
+    ("d3q19-constant-shell-guo", Config(lattice=LatticeType.D3Q19, nx=16, ny=12, nz=10,
+                                        force=ForceType.ConstantShell,
+                                        amplitude=(1e-2, 2e-2, -1e-2), tau=0.6, **SPEC)),
+    ("d3q27-energy-removal", Config(lattice=LatticeType.D3Q27, nx=12, ny=10, nz=8,
+                                    force=ForceType.EnergyRemoval, tau=0.7, **SPEC)),
+    ("d3q27-turbulent-odd", Config(lattice=LatticeType.D3Q27, nx=9, ny=10, nz=7,
+                                   forcing_scheme=ForcingSchemeType.ExactDifferenceMethod,
+                                   force=ForceType.Turbulent2D, amplitude=(1e-2, 0, 1e-2),
+                                   tau=0.8, **SPEC)),
+]
+IDS = [c[0] for c in CASES]
+
+
+def oracle_cfg(c):
+    return o.Config(lattice=int(c.lattice), collision=int(c.collision),
+                    forcing_scheme=int(c.forcing_scheme), force=int(c.force),
+                    equilibrium=int(c.equilibrium), nx=c.nx, ny=c.ny, nz=c.nz, tau=c.tau,
+                    amplitude=tuple(c.amplitude), wavelength=tuple(c.wavelength))
+
+
+def spec_of(c):
+    return dict(kmin=c.force_kmin, kmax=c.force_kmax, removal_amplitude=c.removal_amplitude,
+                removal_kmin=c.removal_kmin, removal_kmax=c.removal_kmax)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("name,c", CASES, ids=IDS)
+def test_force_field_matches_oracle(name, c):
+    print(f"case spectral force field {name}", flush=True)
+    oc = oracle_cfg(c)
+    u0 = o.noise_velocity(oc, 7, 2e-2)
+    with Algorithm(c) as a:
+        a.init_noise(1.0, 2e-2, 7)
+        a.iterate(1, store=True)
+        F1 = a.store_fields()[3]
+        Fo = o.spectral_force_field(oc, spec_of(c), np.ones(oc.cells), u0)
+        assert np.max(np.abs(Fo)) > 0
+        assert rel(F1, Fo) <= 1e-12, rel(F1, Fo)
+        # the next field comes from the fields of that storing step
+        rho, u = a.store_fields()[:2]
+        a.iterate(1, store=True)
+        F2 = a.store_fields()[3]
+        Fo2 = o.spectral_force_field(oc, spec_of(c), rho, u)
+        assert rel(F2, Fo2) <= 1e-12, rel(F2, Fo2)
+
+
+@pytest.mark.parametrize("name,c", CASES, ids=IDS)
+def test_step_with_the_gpu_field_matches_oracle(name, c):
+    print(f"case spectral force step {name}", flush=True)
+    oc = oracle_cfg(c)
+    with Algorithm(c) as a:
+        a.init_noise(1.0, 2e-2, 3)
+        for _ in range(4):
+            a.iterate(1, store=True)
+        f4 = a.populations()
+        al4 = a.alpha() if c.collision == CollisionType.ELBM else None
+        a.iterate(1, store=True)
+        f5 = a.populations()
+        F5 = a.store_fields()[3]
+    fo, rho, u, Fo = o.step_field(oc, f4, al4, F5)
+    np.testing.assert_array_equal(Fo, F5)
+    if c.collision == CollisionType.BGK:
+        np.testing.assert_array_equal(f5, fo)  # FP64 reference order: bitwise
+    else:
+        assert np.max(np.abs(f5 - fo)) <= 1e-12
+
+
+@pytest.mark.parametrize("store_every", [1, 5])
+@pytest.mark.parametrize("name,c", CASES, ids=IDS)
+def test_runs_match_oracle(name, c, store_every):
+    print(f"case spectral force run {name} store_every={store_every}", flush=True)
+    oc = oracle_cfg(c)
+    u0 = o.noise_velocity(oc, 11, 2e-2)
+    f0 = o.init_equilibrium(oc, np.ones(oc.cells), u0)
+    fo, alo, rho_o, u_o, F_o = o.run_field_force(oc, f0, 10, np.ones(oc.cells), u0, spec_of(c),
+                                                 store_every=store_every)
+    with Algorithm(c) as a:
+        a.init_noise(1.0, 2e-2, 11)
+        for _ in range(10 // store_every):
+            a.iterate(store_every, store=True)
+        f = a.populations()
+        rho, u, al, F = a.store_fields()
+    assert rel(F, F_o) <= 1e-11, rel(F, F_o)
+    # BGK differs only through the force fields' rounding; ELBM adds the
+    # device log's (the entropic parity bar is 1e-12)
+    tol = 1e-12 if c.collision == CollisionType.ELBM else 1e-13
+    assert np.max(np.abs(f - fo)) <= tol, np.max(np.abs(f - fo))
+    assert rel(u, u_o) <= 1e-10, rel(u, u_o)
+
+
+def test_fp32_compute_tracks_fp64():
+    print("case spectral force fp32", flush=True)
+    c64 = CASES[0][1]
+    c32 = Config(**{**c64.__dict__, "storage": Precision.F32, "compute": Precision.F32,
+                    "shifted": True})
+    out = []
+    for c in (c64, c32):
+        with Algorithm(c) as a:
+            a.init_noise(1.0, 2e-2, 5)
+            for _ in range(4):
+                a.iterate(5, store=True)
+            out.append(a.store_fields())
+    assert rel(out[1][3], out[0][3]) <= 1e-5   # force field
+    assert rel(out[1][1], out[0][1]) <= 1e-4   # velocity

@@ -110,8 +110,10 @@ typedef struct {
    *  CONSTANT_SHELL: F = curl psi, psi_hat = amplitude on the shell
    *    (MakeIncompressible, Transformer.h:301-389), set once;
    *  ENERGY_REMOVAL: F_d = -removal_amplitude_d (rho u_d) restricted to the
-   *    removal shell, from the fields of the last storing step (the initial
-   *    fields before any), recomputed before every step;
+   *    removal shell, from the fields of the last storing step (before any:
+   *    the initialiser's rho0 and u0, or after lbm_set_populations the
+   *    restart fields rho = sum f_i, u = sum c_i f_i, undivided as in
+   *    Initialize.h:124-145), recomputed before every step;
    *  TURBULENT_2D: the sum of both (injection: amplitude, force_kmin..kmax). */
   int force_kmin, force_kmax;     /* forcekMin, forcekMax (Input_prod_2d.in:51-52) */
   double removal_amplitude[3];    /* removalForceAmplitude (Input_prod_2d.in:54) */

@@ -379,6 +379,43 @@ hipError_t launch_fill(int type, void* dst, double value, int N,
   return hipGetLastError();
 }
 
+// Restart fields (Initialize.h:124-145): density = sum_i f_i and velocity =
+// sum_i c_i f_i -- the reference does not divide by the density there --
+// accumulated one population at a time in index order.
+__global__ void k_restart_fields(double* __restrict__ rho, double* __restrict__ u,
+                                 const double* __restrict__ f, int N, int D, int cx, int cy,
+                                 int cz, int first) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  const double v = f[x];
+  const int c[3] = {cx, cy, cz};
+  rho[x] = first ? v : rho[x] + v;
+  for (int d = 0; d < D; ++d) {
+    const double m = (double)c[d] * v;
+    u[(size_t)d * N + x] = first ? m : u[(size_t)d * N + x] + m;
+  }
+}
+
+hipError_t launch_restart_fields(int lattice, double* rho, double* u, const double* f, int i,
+                                 int N, hipStream_t st) {
+  int c[3] = {0, 0, 0}, D = 3;
+  switch (lattice) {
+    case (int)LatticeType::D2Q9:
+      D = 2;
+      for (int d = 0; d < 3; ++d) c[d] = Lattice<LatticeType::D2Q9>::C[i][d];
+      break;
+    case (int)LatticeType::D3Q19:
+      for (int d = 0; d < 3; ++d) c[d] = Lattice<LatticeType::D3Q19>::C[i][d];
+      break;
+    default:
+      for (int d = 0; d < 3; ++d) c[d] = Lattice<LatticeType::D3Q27>::C[i][d];
+      break;
+  }
+  hipLaunchKernelGGL(k_restart_fields, dim3((N + 255) / 256), dim3(256), 0, st, rho, u, f, N, D,
+                     c[0], c[1], c[2], i == 0 ? 1 : 0);
+  return hipGetLastError();
+}
+
 int reduce_blocks(int N) { return (N + RED_CELLS - 1) / RED_CELLS; }
 
 hipError_t launch_reduce(const double* rho, const double* u, int nx, int ny,

@@ -48,6 +48,12 @@ hipError_t launch_scalar_to_f64(int type, double* dst, const void* src, int N,
 hipError_t launch_fill(int type, void* dst, double value, int N,
                        hipStream_t st);
 
+// Restart fields from population i (FP64, unshifted) of a restart:
+// rho (=|+=) f_i, u_d (=|+=) c_id f_i, starting over at i = 0
+// (Initialize.h:124-145).
+hipError_t launch_restart_fields(int lattice, double* rho, double* u, const double* f, int i,
+                                 int N, hipStream_t st);
+
 // Deterministic FP64 diagnostics: out[0] = sum rho, out[1] = sum 1/2 rho
 // |u|^2, out[2] = sum 1/2 |curl u|^2.  partial must hold 3 * reduce_blocks(N)
 // doubles.

@@ -834,6 +834,11 @@ int lbm_set_populations(lbm_handle* h, const double* f) {
     const double s = h->cfg.storage == LBM_F16 ? h->scale[i] : 1.0;
     HIP_TRY(h, launch_pop_from_f64(h->cfg.storage, dst + (size_t)i * h->N * h->elem,
                                    h->staging, h->N, w, s, h->stream));
+    // the fields a restart derives from the populations (Initialize.h:124-145),
+    // read by EnergyRemoval until the first storing step
+    if (h->cfg.force == LBM_FORCE_ENERGY_REMOVAL || h->cfg.force == LBM_FORCE_TURBULENT_2D)
+      HIP_TRY(h, launch_restart_fields(h->cfg.lattice, h->fields, h->fields + h->N, h->staging,
+                                       i, h->N, h->stream));
   }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return LBM_OK;

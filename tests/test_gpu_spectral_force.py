@@ -133,3 +133,28 @@ def test_fp32_compute_tracks_fp64():
             out.append(a.store_fields())
     assert rel(out[1][3], out[0][3]) <= 1e-5   # force field
     assert rel(out[1][1], out[0][1]) <= 1e-4   # velocity
+
+
+def test_restart_fields_feed_energy_removal():
+    """After lbm_set_populations (a restart), EnergyRemoval reads the fields
+    the reference's restart derives from the populations (Initialize.h:
+    124-145): density sum_i f_i and velocity sum_i c_i f_i, undivided."""
+    print("case spectral force restart fields", flush=True)
+    c = CASES[0][1]
+    oc = oracle_cfg(c)
+    with Algorithm(c) as a:
+        a.init_noise(1.0, 2e-2, 13)
+        a.iterate(5, store=True)
+        f5 = a.populations()
+    with Algorithm(c) as b:
+        b.set_populations(f5)
+        b.iterate(3, store=True)  # all three steps use the restart fields
+        f8 = b.populations()
+    cv = o.velocities(oc.lattice).astype(np.float64)  # [Q, D]
+    rho0 = f5.sum(axis=0)
+    mom0 = cv.T @ f5
+    fo = o.run_field_force(oc, f5, 3, rho0, mom0, spec_of(c), store_every=3)[0]
+    assert np.max(np.abs(f8 - fo)) <= 1e-13, np.max(np.abs(f8 - fo))
+    # dividing by the density instead would be visible at this tolerance
+    fd = o.run_field_force(oc, f5, 3, rho0, mom0 / rho0, spec_of(c), store_every=3)[0]
+    assert np.max(np.abs(fd - fo)) > 1e-12

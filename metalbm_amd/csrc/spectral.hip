@@ -109,14 +109,17 @@ __host__ __device__ inline int wavenumber(int i, int n) { return i <= n / 2 ? i 
 struct ModeIndex {
   int i[3];
 };
-__host__ __device__ inline ModeIndex mode_index(int64_t m, int D, const int* n) {
+// 32-bit unsigned arithmetic: M <= N < 2^31 (64-bit division is emulated
+// on the GPU and dominated the per-mode kernels).
+__host__ __device__ inline ModeIndex mode_index(int64_t m64, int D, const int* n) {
   ModeIndex r{{0, 0, 0}};
-  const int nh = n[D - 1] / 2 + 1;
+  const unsigned m = (unsigned)m64;
+  const unsigned nh = (unsigned)(n[D - 1] / 2 + 1);
   r.i[D - 1] = (int)(m % nh);
-  int64_t rest = m / nh;
+  unsigned rest = m / nh;
   if (D == 3) {
-    r.i[1] = (int)(rest % n[1]);
-    rest /= n[1];
+    r.i[1] = (int)(rest % (unsigned)n[1]);
+    rest /= (unsigned)n[1];
   }
   r.i[0] = (int)rest;
   return r;
@@ -213,13 +216,14 @@ __device__ inline int shell_of(long long k2, bool physical) {
 }
 
 constexpr int kShellWindow = 2048;  // shells binned per pass (LDS: 4 waves x 16 KB)
-constexpr int kShellBlocks = 1024;
+constexpr int kShellBlocks = 4096;  // 1024 left the binning latency-bound (171 us at 256^3)
 
 // Shell sums of c(m) sum_d |hat_d(m)|^2 over the shells [s_lo, s_lo + cnt),
 // c = 1/2 on the k_last = 0 plane and, when physical, on the k_last = n/2
 // plane of an even last axis.  Modes are read in storage order (coalesced);
-// each wavefront adds its lanes' values into its own LDS bins one lane at a
-// time in lane order, so the sums do not depend on timing; the block then
+// each wavefront sums its runs of equal shells by a segmented scan and adds
+// the run sums into its own LDS bins one lane at a time in lane order, so the
+// sums do not depend on timing; the block then
 // writes the sum of its wavefronts' bins (in wavefront order) to
 // partial[block][cnt].
 __global__ void __launch_bounds__(kBlock) k_shell_bins(const double2* __restrict__ hat, Dims g,
@@ -259,7 +263,25 @@ __global__ void __launch_bounds__(kBlock) k_shell_bins(const double2* __restrict
         val = c * e;
       }
     }
-    unsigned long long todo = __ballot(key >= 0);
+    // Consecutive modes along the last axis fall in nondecreasing shells, so
+    // the wavefront's keys form runs: a segmented scan (fixed shuffle order)
+    // sums each run into its last lane, and only those lanes add to the bins.
+    const int prev_key = __shfl_up(key, 1);
+    const bool head = lane == 0 || prev_key != key;
+    int start = head ? lane : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(start, off);
+      if (lane >= off && o > start) start = o;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double o = __shfl_up(val, off);
+      if (lane - off >= start) val += o;
+    }
+    const int next_key = __shfl_down(key, 1);
+    const bool tail = lane == 63 || next_key != key;
+    unsigned long long todo = __ballot(tail && key >= 0);
     while (todo) {
       const int t = __ffsll((long long)todo) - 1;
       todo &= todo - 1;
@@ -360,6 +382,7 @@ __global__ void k_to_f32(const double* __restrict__ src, int64_t n, float* __res
 template <int D>
 constexpr int modes_per_pass() { return 8; }
 constexpr int kMaxDirectModes = 32;
+constexpr int kDftBlocks = 1024;  // 4096 measured 33% slower at D2Q9 2048^2
 
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -654,7 +677,7 @@ int direct_setup(lbm_handle* h, State& s, const lbm::ForceField& ff, const Force
   }
   if (modes.empty() || (int)modes.size() > kMaxDirectModes) return LBM_OK;
   s.nmodes = (int)modes.size();
-  s.dft_blocks = (int)std::min<int64_t>(kShellBlocks, (s.N + kBlock - 1) / kBlock);
+  s.dft_blocks = (int)std::min<int64_t>(kDftBlocks, (s.N + kBlock - 1) / kBlock);
   std::vector<double2> tw;
   for (int d = 0; d < s.D; ++d)
     for (int m = 0; m < s.n[d]; ++m) {

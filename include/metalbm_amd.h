@@ -64,8 +64,11 @@ enum { LBM_F64 = 0, LBM_F32 = 1, LBM_F16 = 2 };
 
 /* Kernel shape of the BGK step. */
 enum {
-  LBM_KERNEL_AUTO = 0,    /* vector kernel when the contiguous extent allows
-                             (nz in 3D, ny in 2D, divisible by V) */
+  LBM_KERNEL_AUTO = 0,    /* the vector kernel for D3Q19 with FP32 storage and
+                             for FP64 storage with FP32 compute, when the
+                             contiguous extent (nz in 3D, ny in 2D) is
+                             divisible by V; else one cell per lane (the
+                             faster shape measured for each) */
   LBM_KERNEL_CELL = 1,    /* one cell per lane */
   LBM_KERNEL_VECTOR = 2   /* V consecutive cells per lane, aligned vector
                              accesses; needs nz (3D) / ny (2D) divisible by

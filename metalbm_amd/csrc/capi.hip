@@ -502,7 +502,17 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   KernelKey key{cfg->lattice, cfg->collision, src_of(cfg->forcing_scheme),
                 cfg->equilibrium, cfg->storage, h->ctype, cfg->shifted ? 1 : 0};
   h->step = registry().find(key);
-  if (cfg->collision == LBM_BGK && cfg->kernel != LBM_KERNEL_CELL) {
+  // LBM_KERNEL_AUTO takes the vector kernel only where it measured faster
+  // than one cell per lane (MI355X, 100-step runs): D3Q19 with FP32 storage
+  // at 256^3 (+2%) and FP64 storage with FP32 compute (D3Q27 384^3, +2%).
+  // One cell per lane won for D3Q19 FP64 (+12%), D3Q27 FP16 (+10%) and FP32
+  // (+4%) storage, and D2Q9 128^2 FP64 (+48%): more wavefronts in flight.
+  const bool vector_preferred =
+      (cfg->lattice == LBM_D3Q19 && cfg->storage == LBM_F32) ||
+      (cfg->storage == LBM_F64 && cfg->compute == LBM_F32);
+  if (cfg->collision == LBM_BGK &&
+      (cfg->kernel == LBM_KERNEL_VECTOR ||
+       (cfg->kernel == LBM_KERNEL_AUTO && vector_preferred))) {
     // Vector width V, in this order (measured on D3Q27 384^3 rows):
     //  1. 16-byte lanes (V sizeof(S) = 16) even if the last 64-lane segment
     //     of a row is partial (FP32 V=4 split 64 + 32 lanes: +3% over V=2);

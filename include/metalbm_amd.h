@@ -85,7 +85,8 @@ enum {
                              residual_tol, or the corrected point when its
                              Taylor remainder bound is <= residual_tol; else
                              the reference Newton from the stored alpha
-                             decides (with its alpha = 2 fallback).  The
+                             decides (with its alpha = 2 fallback), in a
+                             second launch over the cells so deferred.  The
                              alpha state is kept as for NEWTON. */
   LBM_ALPHA_WARM_RESIDUAL = 2  /* the same with the stored alpha as the start:
                              the method without the network */

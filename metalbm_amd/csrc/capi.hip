@@ -86,6 +86,9 @@ struct lbm_handle {
   double* ffield = nullptr;
   float* ffield32 = nullptr;
   bool ffield_built = false;
+  // alpha modes 1 and 2: count and list of the cells the correction defers
+  unsigned* defer_count = nullptr;
+  int* defer_list = nullptr;
 };
 
 namespace {
@@ -353,7 +356,8 @@ void release(lbm_handle* h) {
   for (void* p : {h->f[0], h->f[1], h->alpha, (void*)h->fields,
                   (void*)h->staging, (void*)h->reduce_buf, h->ftab[0],
                   h->ftab[1], h->ftab[2], (void*)h->stats, (void*)h->mlp, h->mlp_packed,
-                  (void*)h->mlp_m, (void*)h->mlp_v, (void*)h->log_x,
+                  (void*)h->mlp_m, (void*)h->mlp_v, (void*)h->defer_count,
+                  (void*)h->defer_list, (void*)h->log_x,
                   (void*)h->log_alpha, (void*)h->log_count})
     if (p) (void)hipFree(p);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -379,6 +383,8 @@ StepArgs make_args(const lbm_handle* h) {
   }
   a.alpha_state = h->alpha;
   a.stats = h->stats;
+  a.defer_count = h->defer_count;
+  a.defer_list = h->defer_list;
   a.alpha_mode = h->alpha_mode;
   a.residual_tol = h->residual_tol;
   a.mlp_packed = h->mlp_packed;
@@ -906,6 +912,10 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
                 "lbm_mlp_set_normalization");
   if (h->cfg.collision == LBM_ELBM && h->alpha_mode == LBM_ALPHA_LEARNED)
     if (int rc = pack_mlp(h)) return rc;
+  if (h->cfg.collision == LBM_ELBM && h->alpha_mode != LBM_ALPHA_NEWTON && !h->defer_list) {
+    HIP_TRY(h, hipMalloc(&h->defer_count, sizeof(unsigned)));
+    HIP_TRY(h, hipMalloc(&h->defer_list, (size_t)h->N * sizeof(int)));
+  }
   StepArgs a = make_args(h);
   // Vector kernel: one wavefront per 64 V-cell segment, 4 per workgroup.
   const int grid = h->vec ? (h->rows * h->tiles_per_row + 3) / 4

@@ -257,6 +257,13 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // on one MI355X): D2Q9 learned three (168 VGPRs, 2 spilled; 17.3 k MLUPS vs
 // 13.8 k at two); D3Q27 two (reference Newton 2.72 k vs 2.03 k MLUPS at one
 // wavefront with 289 registers; certified correction +16%, learned +9%).
+// Resets the deferred-solve counter before the correction launch (a kernel,
+// not a memset, so that the step can be captured in a graph).
+template <int UNUSED = 0>
+__global__ void k_defer_reset(unsigned* count) {
+  if (threadIdx.x == 0) *count = 0u;
+}
+
 template <class L, int MODE>
 constexpr int elbm_min_waves() {
   if (L::Type == LatticeType::D2Q9 && MODE == 1) return 3;
@@ -281,9 +288,22 @@ __global__ void __launch_bounds__(BLOCK)
   char* wave_lds = s_lds + (threadIdx.x >> 6) * Lay::WAVE;
   if constexpr (MODE == 1)
     copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, wave_lds);
-  const CellPos c = cell_of_thread<L, BLOCK>(a);
+  // MODE 3: the reference Newton for the cells the correction of MODE 1 or 2
+  // deferred (one thread per listed cell).
+  CellPos c;
+  if constexpr (MODE == 3) {
+    const int t = blockIdx.x * BLOCK + threadIdx.x;
+    if (t < (int)*a.defer_count) {
+      c = cell_from_index<L>(a, a.defer_list[t]);
+    } else {
+      c = CellPos{};
+      c.active = false;
+    }
+  } else {
+    c = cell_of_thread<L, BLOCK>(a);
+  }
   int branch = -1, its = 0;
-  bool first = false, solve = false;
+  bool first = false, solve = false, deferred = false;
   int cell = 0;
   C f[Q], fneq[Q], eqv[Q], F[3];
   C rho = C(1), u[3] = {C(0), C(0), C(0)}, u2 = C(0), drho = C(0);
@@ -341,22 +361,23 @@ __global__ void __launch_bounds__(BLOCK)
     C* __restrict__ astate = static_cast<C*>(a.alpha_state);
     if (solve) {
       const Fp64View<L, C, SH> p{f, fneq};
-      if constexpr (MODE == 0) {
+      if constexpr (MODE == 0 || MODE == 3) {
         alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, its);
       } else {
         // MODE 1 starts from the network's alpha, MODE 2 from the stored one;
-        // one certified correction, else the reference Newton decides.
+        // one certified correction, else the reference Newton decides -- in
+        // the MODE 3 launch that follows, from the untouched stored alpha.
         if constexpr (MODE == 2) start = double(astate[cell]);
         int ev = 0;
+        its = ev;
         if (residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev)) {
           branch = 2;
           first = true;
-          its = ev;
         } else {
-          int nit = 0;
-          alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, nit);
-          its = ev + nit;
+          deferred = true;
+          a.defer_list[atomicAdd(a.defer_count, 1u)] = cell;
         }
+        its = ev;
       }
       // The sample log records the reference Newton's solves (MODE 0 only).
       if constexpr (Shape::NIN > 0 && MODE == 0) {
@@ -373,6 +394,7 @@ __global__ void __launch_bounds__(BLOCK)
         }
       }
     }
+    if (!deferred) {
     astate[cell] = C(alpha);  // Algorithm.h:106
     const C gc = C(a.guo_coef);
     // tau = 1 / (alpha beta); collide with 1 / tau (Collision.h:240, 254).
@@ -414,14 +436,15 @@ __global__ void __launch_bounds__(BLOCK)
       store_pop<L, i, S, C>(dst + (size_t)i * a.N + cell, out);
     });
     if (a.store) store_fields<L, C>(a, cell, FSX, rho, u, F, alpha);
+    }
   }
   if (a.stats) {
     __syncthreads();
-    if (branch >= 0) {
-      atomicAdd(&s_cnt[branch], 1u);
-      if (its) atomicAdd(&s_cnt[4], (unsigned)its);
-      if (first) atomicAdd(&s_cnt[5], 1u);
-    }
+    // a deferred cell counts its correction's evaluations here and its
+    // branch and Newton iterations in the MODE 3 launch
+    if (branch >= 0) atomicAdd(&s_cnt[branch], 1u);
+    if (its) atomicAdd(&s_cnt[4], (unsigned)its);
+    if (first) atomicAdd(&s_cnt[5], 1u);
     __syncthreads();
     if (threadIdx.x < 6 && s_cnt[threadIdx.x])
       atomicAdd(&a.stats[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);

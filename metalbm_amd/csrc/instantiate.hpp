@@ -19,16 +19,23 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
   else if (a.alpha_mode == 0)
     hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 0>), dim3(grid),
                        dim3(block), 0, st, a, fsx);
-  else if (a.alpha_mode == 2)
-    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 2>), dim3(grid),
-                       dim3(block), 0, st, a, fsx);
-  else if constexpr (MlpShape<L>::NIN > 0) {
-    // The learned solver stages the network in LDS (dynamic shared memory).
-    using M = MfmaLayout<MlpShape<L>::NIN, MlpShape<L>::H1, MlpShape<L>::H2>;
-    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 1>), dim3(grid),
-                       dim3(block), M::BYTES, st, a, fsx);
-  } else {
-    return hipErrorInvalidValue;
+  else {
+    // Modes 1 and 2: the certified correction, then the reference Newton for
+    // the cells it deferred (MODE 3, one thread per possible list entry).
+    hipLaunchKernelGGL((k_defer_reset<0>), dim3(1), dim3(64), 0, st, a.defer_count);
+    if (a.alpha_mode == 2) {
+      hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 2>), dim3(grid),
+                         dim3(block), 0, st, a, fsx);
+    } else if constexpr (MlpShape<L>::NIN > 0) {
+      // The learned solver stages the network in LDS (dynamic shared memory).
+      using M = MfmaLayout<MlpShape<L>::NIN, MlpShape<L>::H1, MlpShape<L>::H2>;
+      hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 1>), dim3(grid),
+                         dim3(block), M::BYTES, st, a, fsx);
+    } else {
+      return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>), dim3((a.N + 255) / 256),
+                       dim3(256), 0, st, a, fsx);
   }
   return hipGetLastError();
 }

@@ -61,6 +61,11 @@ struct StepArgs {
   long long log_cap;
   unsigned log_threshold;  // sample when hash(cell, salt) < threshold
   unsigned log_salt;
+  // Alpha modes 1 and 2: cells whose certified correction fails are listed
+  // here and solved by a second launch running the reference Newton, so the
+  // correction kernel does not carry the Newton's registers.
+  unsigned* defer_count;
+  int* defer_list;  // N entries
 };
 
 // Storage / compute codes (values of LBM_F64 / LBM_F32 / LBM_F16).
@@ -119,6 +124,30 @@ __device__ __forceinline__ CellPos cell_of_thread(const StepArgs& a) {
   } else {
     c.x = row;
     c.y = c.active ? k : 0;
+    c.z = 0;
+  }
+  c.xm = c.x == 0 ? a.nx - 1 : c.x - 1;
+  c.xp = c.x == a.nx - 1 ? 0 : c.x + 1;
+  c.ym = c.y == 0 ? a.ny - 1 : c.y - 1;
+  c.yp = c.y == a.ny - 1 ? 0 : c.y + 1;
+  c.zm = c.z == 0 ? a.nz - 1 : c.z - 1;
+  c.zp = c.z == a.nz - 1 ? 0 : c.z + 1;
+  return c;
+}
+
+// Position of a cell given by its index (the deferred-solve list).
+template <class L>
+__device__ __forceinline__ CellPos cell_from_index(const StepArgs& a, int cell) {
+  CellPos c;
+  c.active = true;
+  if constexpr (L::dimD == 3) {
+    const unsigned u = (unsigned)cell, t = u / (unsigned)a.nz;
+    c.z = (int)(u - t * (unsigned)a.nz);
+    c.y = (int)(t % (unsigned)a.ny);
+    c.x = (int)(t / (unsigned)a.ny);
+  } else {
+    c.y = cell % a.ny;
+    c.x = cell / a.ny;
     c.z = 0;
   }
   c.xm = c.x == 0 ? a.nx - 1 : c.x - 1;

@@ -373,26 +373,12 @@ __device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X
   dg = Sn + S + ds;
 }
 
-// -G''(a) = sum n_i x_i / (1 - a x_i) >= 0, as in residual_terms.
-template <class L, class V>
-__device__ __forceinline__ double residual_curvature(const V& p, const XTerms<L, V>& X,
-                                                     double a) {
-  constexpr int Q = L::dimQ;
-  double s = 0.0;
-  static_for<Q>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    const double xi = X.template get<i>();
-    s += p.template n<i>() * xi * recip_refined(1.0 - a * xi);
-  });
-  return s;
-}
-
 // One residual-checked Newton correction of a starting alpha (G in the log1p
 // form of EntropicStep.h:31-45).  Accepts the start if |G(start)| <= tol,
 // else the Newton point a1 = start - G/G' if it lies in (1, amax) and its
-// Taylor certificate 1/2 s^2 max(|G''(start)|, |G''(a1)|) (G'' is convex in
-// alpha, so its maximum over the step lies at an end point) is <= tol, with a
-// relative rounding allowance.  Returns false -- the caller then runs the
+// Taylor certificate 1/2 s^2 max |G''| over the step is <= tol, with a
+// relative rounding allowance; max |G''| is bounded from the curvature at the
+// start (see below).  Returns false -- the caller then runs the
 // reference Newton -- when the start is outside (1, amax) or neither test
 // passes.  evals: G evaluations made (0 or 1).  p views f and fneq in FP64.
 template <class L, class V>
@@ -433,8 +419,15 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
   const double s = g / dg;
   const double a1 = start - s;
   if (!(a1 > 1.0 && a1 < amax)) return false;  // also a non-finite step
-  const double bound =
-      0.5 * s * s * fmax(h, residual_curvature<L>(p, X, a1)) * (1.0 + 1e-6);
+  // -G'' over the step, without a second pass over the populations: every
+  // term n_i x_i / (1 - a x_i) is >= 0 (n_i x_i = n_i^2 / f_i), and for a
+  // between start and a1, 1/(1 - a x_i) <= (1 + |s| xmax / (1 - am xmax)) /
+  // (1 - start x_i) with am = max(start, a1) -- so -G'' <= h(start) rho.
+  const double am = fmax(start, a1);
+  const double room = 1.0 - am * X.xmax;
+  if (!(room > 0.0)) return false;
+  const double rho = 1.0 + fabs(s) * X.xmax / room;
+  const double bound = 0.5 * s * s * h * rho * (1.0 + 1e-6);
   if (!(bound <= tol)) return false;
   alpha = a1;
   return true;

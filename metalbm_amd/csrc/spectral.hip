@@ -400,21 +400,33 @@ __device__ inline double2 power_of(double2 e1, int k) {
   return k < 0 ? make_double2(r.x, -r.y) : r;
 }
 
-// First harmonics e^{-2 pi i x_d / n_d} of a cell from tw = [nx | ny | nz].
+// First and second harmonics e^{-2 pi i x_d / n_d}, e^{-4 pi i x_d / n_d} of
+// a cell from tw = [nx | ny | nz], shared by all the modes.
 struct CellPhases {
-  double2 e[3];
+  double2 e[3], e2[3];
 };
 __device__ inline void cell_phases(const double2* __restrict__ tw, const Dims& g, const int* i,
                                    CellPhases& c) {
   c.e[0] = tw[i[0]];
   c.e[1] = tw[g.n[0] + i[1]];
   c.e[2] = g.D == 3 ? tw[g.n[0] + g.n[1] + i[2]] : make_double2(1.0, 0.0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) c.e2[d] = cmul(c.e[d], c.e[d]);
+}
+
+// e^{-i k theta} along one axis: the shared harmonics for |k| <= 2 (uniform
+// branches: one mode at a time), repeated products beyond.
+__device__ inline double2 axis_phase(double2 e1, double2 e2, int k) {
+  const int a = k < 0 ? -k : k;
+  const double2 r = a == 0 ? make_double2(1.0, 0.0)
+                           : (a == 1 ? e1 : (a == 2 ? e2 : power_of(e1, a)));
+  return k < 0 ? make_double2(r.x, -r.y) : r;
 }
 
 // e^{-i k.x} of mode m (signed wavenumbers) at the cell.
 __device__ inline double2 phase_of(const CellPhases& c, const Dims& g, int4 m) {
-  double2 p = cmul(power_of(c.e[0], m.x), power_of(c.e[1], m.y));
-  if (g.D == 3) p = cmul(p, power_of(c.e[2], m.z));
+  double2 p = cmul(axis_phase(c.e[0], c.e2[0], m.x), axis_phase(c.e[1], c.e2[1], m.y));
+  if (g.D == 3) p = cmul(p, axis_phase(c.e[2], c.e2[2], m.z));
   return p;
 }
 

@@ -452,7 +452,6 @@ __global__ void __launch_bounds__(kBlock) k_mode_dft_partial(
     const double* __restrict__ rho, const double* __restrict__ u, Dims g, int64_t N,
     const int4* __restrict__ modes, int nmodes, const double2* __restrict__ tw,
     double2* __restrict__ partial) {
-  __shared__ double lds[kBlock];
   const int j0 = blockIdx.y * MP;
   const int nm = min(MP, nmodes - j0);
   double2 acc[MP][D];
@@ -482,19 +481,33 @@ __global__ void __launch_bounds__(kBlock) k_mode_dft_partial(
       }
     }
   }
+  // wavefront butterfly sums (no barrier), then the block's wavefronts in
+  // order: one barrier instead of one per accumulator
+  __shared__ double2 red[kBlock / 64][MP][D];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < MP; ++j) {
-    if (j < nm) {  // uniform across the block
+  for (int j = 0; j < MP; ++j)
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const double re = block_sum(acc[j][d].x, lds);
-        __syncthreads();
-        const double im = block_sum(acc[j][d].y, lds);
-        __syncthreads();
-        if (threadIdx.x == 0)
-          partial[((int64_t)blockIdx.x * nmodes + j0 + j) * D + d] = make_double2(re, im);
+    for (int d = 0; d < D; ++d) {
+      double re = acc[j][d].x, im = acc[j][d].y;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off);
+        im += __shfl_xor(im, off);
       }
+      if (lane == 0) red[wave][j][d] = make_double2(re, im);
     }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < nm * D) {
+    const int j = t / D, d = t - j * D;
+    double2 sum = red[0][j][d];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+      sum.x += red[w][j][d].x;
+      sum.y += red[w][j][d].y;
+    }
+    partial[((int64_t)blockIdx.x * nmodes + j0 + j) * D + d] = sum;
   }
 }
 
@@ -537,11 +550,18 @@ __global__ void k_force_synth(const double* __restrict__ finj, const double2* __
   double f[3] = {0.0, 0.0, 0.0};
   if (finj)
     for (int d = 0; d < g.D; ++d) f[d] = finj[d * N + x];
-  for (int j = 0; j < nmodes; ++j) {
-    const double2 e = phase_of(cp, g, modes[j]);  // e^{-i k.x}
-    for (int d = 0; d < g.D; ++d) {
-      const double2 c = coef[j * g.D + d];
-      f[d] += c.x * e.x + c.y * e.y;  // Re(c e^{+i k.x})
+  // constant trip count (uniform guard) so the mode and coefficient loads of
+  // all iterations can be issued ahead
+#pragma unroll
+  for (int j = 0; j < kMaxDirectModes; ++j) {
+    if (j < nmodes) {
+      const double2 e = phase_of(cp, g, modes[j]);  // e^{-i k.x}
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (d < g.D) {
+          const double2 c = coef[j * g.D + d];
+          f[d] += c.x * e.x + c.y * e.y;  // Re(c e^{+i k.x})
+        }
     }
   }
   for (int d = 0; d < g.D; ++d) F[d * N + x] = (OUT)f[d];

@@ -468,6 +468,44 @@ def diagnostics_bench(K: int, W: int, n: int, cpu: bool) -> dict:
     return out
 
 
+def hdf5_bench(n: int) -> dict:
+    """Field and distribution files in the reference's HDF5 layout (FP64,
+    FFTW-padded last extent) for the D3Q19 n^3 FP32 workload: wall time of
+    lbm_write_fields_hdf5, lbm_write_distribution_hdf5 and
+    lbm_read_distribution_hdf5 into a temporary directory (device-host copy,
+    padding and libhdf5 included; the files are removed afterwards)."""
+    import shutil
+    import tempfile
+
+    import metalbm_amd as m
+    wl = WORKLOADS[DEFAULT]
+    cfg = make_config({**wl["cfg"], "nx": n, "ny": n, "nz": n})
+    d = tempfile.mkdtemp(prefix="metalbm_hdf5_")
+    out = {"metric": "HDF5 field / distribution files, reference layout", "unit": "s",
+           "config": {"workload": f"d3q19-bgk-{n}-f32 fields and populations", "cells": cfg.cells}}
+    try:
+        with m.Algorithm(cfg) as a:
+            init(a, wl["init"])
+            a.iterate(2, store=True)
+            a.synchronize()
+            t0 = time.perf_counter()
+            fpath = a.write_fields(d, 2)
+            t1 = time.perf_counter()
+            dpath = a.write_distribution(d, 2)
+            t2 = time.perf_counter()
+            a.read_distribution(d, 2)
+            t3 = time.perf_counter()
+            fb, db = os.path.getsize(fpath), os.path.getsize(dpath)
+        out.update({"write_fields_s": round(t1 - t0, 3), "field_file_bytes": fb,
+                    "write_distribution_s": round(t2 - t1, 3),
+                    "read_distribution_s": round(t3 - t2, 3), "distribution_file_bytes": db,
+                    "write_distribution_GBps": round(db / (t2 - t1) / 1e9, 2),
+                    "read_distribution_GBps": round(db / (t3 - t2) / 1e9, 2)})
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def traffic_of(name):
     """HBM bytes per launch of the workload's step kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_<name>.txt,
@@ -503,7 +541,14 @@ def main(argv=None) -> int:
                     help="time lbm_vorticity + lbm_spectra on D3Q19 n^3 stored fields "
                          "(prints its own JSON line)")
     ap.add_argument("--diagnostics-n", type=int, default=256)
+    ap.add_argument("--hdf5", action="store_true",
+                    help="time the HDF5 field / distribution files of D3Q19 n^3 "
+                         "(prints its own JSON line)")
+    ap.add_argument("--hdf5-n", type=int, default=128)
     args = ap.parse_args(argv)
+    if args.hdf5:
+        print(json.dumps(hdf5_bench(args.hdf5_n)), flush=True)
+        return 0
     if args.diagnostics:
         print(json.dumps(diagnostics_bench(args.steps or 20, args.warmup, args.diagnostics_n,
                                            not args.no_cpu_baseline)), flush=True)

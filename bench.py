@@ -356,7 +356,9 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
     to <= 1e-12 on every tested configuration).  E and Omega every `every` steps
     (E = (1/N) sum 1/2 rho |u_h|^2, Omega = (1/N) sum 1/2 |curl u_h|^2 by
     central differences).  The learned run logs (features, alpha) over its
-    first `every` steps with the Newton solver, trains, then switches."""
+    first `every` steps with the Newton solver, trains, then switches; the
+    retrained run also logs the Newton solves of the cells its correction
+    defers and retrains on them every 5 intervals."""
     import numpy as np
 
     import metalbm_amd as m
@@ -368,7 +370,8 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
                                    shifted=False), m.AlphaMode.NEWTON),
             "f32-newton": (f32, m.AlphaMode.NEWTON),
             "f32-warm-residual": (f32, m.AlphaMode.WARM_RESIDUAL),
-            "f32-learned": (f32, m.AlphaMode.LEARNED)}
+            "f32-learned": (f32, m.AlphaMode.LEARNED),
+            "f32-learned-retrained": (f32, m.AlphaMode.LEARNED)}
     urms = 0.1 / 3 ** 0.5
     series, stats = {}, {}
     for name, (prec, mode) in runs.items():
@@ -387,6 +390,11 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
                 else:
                     if k == 0:
                         a.set_alpha_mode(mode, 1e-10)
+                    if name == "f32-learned-retrained" and k % 5 == 1:
+                        # log the deferred Newton solves, retrain every 5 intervals
+                        if k > 1:
+                            a.mlp_train(1000, 4096, 1e-3, seed=k)
+                        a.sample_log_enable(500000, 1e-2, k)
                     a.iterate(every, store=True)
                 out.append(a.reduce()[1:])
                 if (k + 1) % 10 == 0:

@@ -250,8 +250,9 @@ int lbm_mlp_gradient(lbm_handle *h, const float *x, const double *alpha, int n,
  * accumulation, FP32 biases). */
 int lbm_mlp_predict(lbm_handle *h, const float *x, int n, double *alpha);
 /* Logs (features, alpha) of a `fraction` of the cells the reference Newton
- * solves during subsequent LBM_ALPHA_NEWTON steps, up to `capacity` samples,
- * into device memory (steps in the other alpha modes log nothing). */
+ * solves during subsequent steps, up to `capacity` samples, into device
+ * memory: every solve in LBM_ALPHA_NEWTON, in the other modes the solves of
+ * the cells their correction could not certify (data to retrain on). */
 int lbm_sample_log_enable(lbm_handle *h, int64_t capacity, double fraction,
                           uint32_t salt);
 int lbm_sample_log_disable(lbm_handle *h);

@@ -378,8 +378,10 @@ __global__ void __launch_bounds__(BLOCK)
         }
         its = ev;
       }
-      // The sample log records the reference Newton's solves (MODE 0 only).
-      if constexpr (Shape::NIN > 0 && MODE == 0) {
+      // The sample log records the reference Newton's solves: every solve in
+      // MODE 0, the solves the correction deferred in MODE 3 (the cells the
+      // network or the stored alpha missed -- data to retrain on).
+      if constexpr (Shape::NIN > 0 && (MODE == 0 || MODE == 3)) {
         if (a.log_cap > 0 && branch == 2 &&
             sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {
           const unsigned long long k = atomicAdd(a.log_count, 1ull);

@@ -49,6 +49,7 @@ struct lbm_handle {
   void* alpha = nullptr;    // ELBM warm-start field (compute type)
   double* fields = nullptr; // rho[N], uh[D N], alpha[N], F[D N]
   bool fields_valid = false;
+  uint64_t fields_version = 0;  // bumped whenever the fields are rewritten
   double* staging = nullptr;   // N doubles
   double* reduce_buf = nullptr;
   void* ftab[3] = {nullptr, nullptr, nullptr};
@@ -336,6 +337,7 @@ int ensure_fields(lbm_handle* h) {
 hipError_t seed_fields(lbm_handle* h, const double* drho, double rho0, const double* du) {
   if (h->cfg.force != LBM_FORCE_ENERGY_REMOVAL && h->cfg.force != LBM_FORCE_TURBULENT_2D)
     return hipSuccess;
+  ++h->fields_version;
   hipError_t e = drho ? hipMemcpyAsync(h->fields, drho, (size_t)h->N * sizeof(double),
                                        hipMemcpyDeviceToDevice, h->stream)
                       : launch_fill(TYPE_F64, h->fields, rho0, h->N, h->stream);
@@ -421,6 +423,7 @@ int lbm::device_fields(lbm_handle* h, DeviceFields& out) {
   out.n[2] = h->cfg.nz;
   out.N = h->N;
   out.stream = h->stream;
+  out.version = h->fields_version;
   return LBM_OK;
 }
 
@@ -852,9 +855,11 @@ int lbm_set_populations(lbm_handle* h, const double* f) {
                                    h->staging, h->N, w, s, h->stream));
     // the fields a restart derives from the populations (Initialize.h:124-145),
     // read by EnergyRemoval until the first storing step
-    if (h->cfg.force == LBM_FORCE_ENERGY_REMOVAL || h->cfg.force == LBM_FORCE_TURBULENT_2D)
+    if (h->cfg.force == LBM_FORCE_ENERGY_REMOVAL || h->cfg.force == LBM_FORCE_TURBULENT_2D) {
+      if (i == 0) ++h->fields_version;
       HIP_TRY(h, launch_restart_fields(h->cfg.lattice, h->fields, h->fields + h->N, h->staging,
                                        i, h->N, h->stream));
+    }
   }
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return LBM_OK;
@@ -945,7 +950,10 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
     h->cur = 1 - h->cur;
     h->steps += 1;
   }
-  if (store_last) h->fields_valid = true;
+  if (store_last) {
+    h->fields_valid = true;
+    ++h->fields_version;
+  }
   return LBM_OK;
 }
 

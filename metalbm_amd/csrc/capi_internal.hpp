@@ -24,6 +24,7 @@ struct DeviceFields {
   int D, n[3];          // extents; n[2] = 1 in 2D
   int64_t N;
   hipStream_t stream;   // the handle's stream
+  uint64_t version;     // changes whenever the stored fields are rewritten
 };
 // LBM_E_STATE (with message) when no storing step has run.
 int device_fields(lbm_handle* h, DeviceFields& out);

@@ -60,6 +60,8 @@ struct State {
   double* partial = nullptr;  // kReduceBlocks + 1
   double* bins = nullptr;     // shell partials: shell_blocks x min(nshells, window)
   int shell_blocks = 0;
+  const double* hat_src = nullptr;  // field whose spectrum hat holds, and its version
+  uint64_t hat_version = 0;
   hipfftHandle inv_d = 0;     // inverse transform of D components (force field)
   bool have_inv_d = false;
   // direct transform of a small removal shell (spectral forces)
@@ -632,14 +634,20 @@ int state_of(lbm_handle* h, const lbm::DeviceFields& f, State*& out) {
 
 // hat = forward transform of the D components at src (copied first: the
 // transform may use its input as scratch).
-int forward(lbm_handle* h, State& s, const double* src, hipStream_t st) {
+int forward(lbm_handle* h, State& s, const double* src, uint64_t version, hipStream_t st) {
+  // vorticity and energy spectrum of the same stored fields share one transform
+  if (s.hat_src == src && s.hat_version == version) return LBM_OK;
+  s.hat_src = nullptr;
   int rc = lbm::record_hip(h, hipMemcpyAsync(s.real, src, sizeof(double) * s.D * s.N,
                                              hipMemcpyDeviceToDevice, st), "field copy");
   if (rc) return rc;
   hipfftResult r = hipfftSetStream(s.fwd, st);
   if (r == HIPFFT_SUCCESS)
     r = hipfftExecD2Z(s.fwd, (hipfftDoubleReal*)s.real, (hipfftDoubleComplex*)s.hat);
-  return r == HIPFFT_SUCCESS ? LBM_OK : fft_fail(h, r, "forward transform");
+  if (r != HIPFFT_SUCCESS) return fft_fail(h, r, "forward transform");
+  s.hat_src = src;
+  s.hat_version = version;
+  return LBM_OK;
 }
 
 int check_scaling(lbm_handle* h, int scaling) {
@@ -659,6 +667,7 @@ namespace {
 int fft_force(lbm_handle* h, State& s, const lbm::ForceField& ff, const ForceSpec& spec,
               double* out) {
   const hipStream_t st = ff.stream;
+  s.hat_src = nullptr;  // hat becomes the force spectrum
   if (!s.have_inv_d) {
     int dims[3] = {s.n[0], s.n[1], s.n[2]};
     const hipfftResult r = hipfftPlanMany(&s.inv_d, s.D, dims, nullptr, 1, (int)s.M, nullptr, 1,
@@ -746,7 +755,7 @@ int direct_setup(lbm_handle* h, State& s, const lbm::ForceField& ff, const Force
 int lbm::update_spectral_force(lbm_handle* h) {
   lbm::ForceField ff;
   if (int rc = lbm::force_field(h, ff)) return rc;
-  lbm::DeviceFields f{ff.rho, ff.u, nullptr, ff.D, {ff.n[0], ff.n[1], ff.n[2]}, ff.N, ff.stream};
+  lbm::DeviceFields f{ff.rho, ff.u, nullptr, ff.D, {ff.n[0], ff.n[1], ff.n[2]}, ff.N, ff.stream, 0};
   State* s = nullptr;
   if (int rc = state_of(h, f, s)) return rc;
   const hipStream_t st = ff.stream;
@@ -816,7 +825,7 @@ int lbm_vorticity(lbm_handle* h, int scaling, double* omega, double* enstrophy) 
   if (int rc = state_of(h, f, s)) return rc;
   const hipStream_t st = f.stream;
   const bool physical = scaling == LBM_SPECTRAL_PHYSICAL;
-  if (int rc = forward(h, *s, f.u, st)) return rc;
+  if (int rc = forward(h, *s, f.u, f.version, st)) return rc;
   const double invN = 1.0 / (double)s->N;
   hipLaunchKernelGGL(k_curl_hat, dim3(grid_of(s->M)), dim3(kBlock), 0, st, s->hat, s->what,
                      dims_of(*s), physical ? 1 : 0, physical ? invN : invN * invN);
@@ -864,7 +873,7 @@ int lbm_spectra(lbm_handle* h, int scaling, double* energy, double* forcing) {
   double* dst[2] = {energy, forcing};
   for (int which = 0; which < 2; ++which) {
     if (!dst[which] || ns == 0) continue;
-    if (int rc = forward(h, *s, src[which], st)) return rc;
+    if (int rc = forward(h, *s, src[which], f.version, st)) return rc;
     for (int lo = 0; lo < ns; lo += kShellWindow) {
       const int cnt = std::min(kShellWindow, ns - lo);
       const size_t lds = sizeof(double) * (kBlock / 64) * cnt;

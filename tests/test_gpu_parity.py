@@ -436,3 +436,35 @@ def test_decaying_turbulence_energy_enstrophy_vs_oracle(oracle):
             f"stats {st}")
         assert rel[:, 0].max() <= be
         assert rel[:, 1].max() <= bw
+
+
+# ---- degenerate extents: a periodic axis of length 1 or 2 pulls from itself
+# or its single neighbour (Boundary.h:45-102 with the halo wrap) ----
+DEGENERATE = [
+    (D2Q9, (1, 16, 1), Precision.F64, Precision.F64, False, 0),
+    (D2Q9, (16, 1, 1), Precision.F64, Precision.F64, False, 0),
+    (D2Q9, (2, 2, 1), Precision.F64, Precision.F64, False, 0),
+    (D3Q19, (1, 1, 8), Precision.F64, Precision.F64, False, 0),
+    (D3Q19, (3, 1, 5), Precision.F64, Precision.F64, False, 0),
+    (D3Q27, (2, 3, 1), Precision.F64, Precision.F64, False, 0),
+    (D3Q27, (1, 2, 8), Precision.F64, Precision.F64, False, 2),        # vector kernel, V=2
+    (D3Q19, (2, 2, 4), Precision.F32, Precision.F32, True, 2),         # vector kernel, V=4
+]
+
+
+@pytest.mark.parametrize("lat,shape,storage,compute,shifted,kernel", DEGENERATE,
+                         ids=[f"{d[0].name}-{'x'.join(map(str, d[1]))}-{d[2].name}-k{d[5]}"
+                              for d in DEGENERATE])
+def test_degenerate_extents(oracle, lat, shape, storage, compute, shifted, kernel):
+    nx, ny, nz = shape
+    cfg = Config(lattice=lat, forcing_scheme=ForcingSchemeType.Guo, force=ForceType.Sinusoidal,
+                 nx=nx, ny=ny, nz=nz, tau=0.6, amplitude=(1e-4, 2e-4, -1e-4),
+                 wavelength=(3.0, 5.0, 7.0), storage=storage, compute=compute, shifted=shifted,
+                 kernel=kernel)
+    say(f"case degenerate {lat.name} {shape} {storage.name} kernel={kernel}")
+    ref, (f, rho, uh, _, F, _) = run_both(oracle, cfg, 7, amp=2e-2)
+    if compute == Precision.F64:
+        assert np.array_equal(f, ref.f)
+        assert np.array_equal(uh, ref.u)
+    else:
+        assert rel_err(f, ref.f) <= 2e-6

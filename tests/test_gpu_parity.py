@@ -468,3 +468,17 @@ def test_degenerate_extents(oracle, lat, shape, storage, compute, shifted, kerne
         assert np.array_equal(uh, ref.u)
     else:
         assert rel_err(f, ref.f) <= 2e-6
+
+
+@pytest.mark.parametrize("lat,shape", [(D2Q9, (1, 16, 1)), (D2Q9, (2, 3, 1)),
+                                       (D3Q27, (2, 1, 6)), (D3Q19, (1, 3, 2))],
+                         ids=["D2Q9-1x16", "D2Q9-2x3", "D3Q27-2x1x6", "D3Q19-1x3x2"])
+def test_degenerate_extents_entropic(oracle, lat, shape):
+    nx, ny, nz = shape
+    cfg = Config(lattice=lat, collision=CollisionType.ELBM, forcing_scheme=ForcingSchemeType.Guo,
+                 force=ForceType.Constant, nx=nx, ny=ny, nz=nz, tau=0.5003,
+                 amplitude=(1e-5, 0, 0))
+    say(f"case degenerate entropic {lat.name} {shape}")
+    ref, (f, rho, uh, al, F, stats) = run_both(oracle, cfg, 6, amp=3e-2)
+    assert np.max(np.abs(f - ref.f)) <= 1e-12
+    assert np.max(np.abs(al - ref.alpha)) <= 1e-9

@@ -1014,8 +1014,10 @@ int lbm_get_elbm_stats(lbm_handle* h, lbm_elbm_stats* out) {
 
 int lbm_reset_elbm_stats(lbm_handle* h) {
   if (!h) return LBM_E_INVALID;
+  // a fill kernel rather than a memset, so a caller may capture this call in
+  // a graph; +0.0 is the all-zero bit pattern of the six 64-bit counters
   if (h->stats)
-    HIP_TRY(h, hipMemsetAsync(h->stats, 0, 6 * sizeof(unsigned long long), h->stream));
+    HIP_TRY(h, launch_fill(TYPE_F64, h->stats, 0.0, 6, h->stream));
   return LBM_OK;
 }
 

@@ -116,9 +116,10 @@ WORKLOADS = {
                    lr=2e-3)),
     "d2q9-bgk-turbulent2d-2048-f32": dict(
         desc="D2Q9 SRT BGK 2048^2, tau 0.6, EDM + Turbulent2D spectral force (injection "
-             "amplitude 5e-4 on shells 5..7, energy removal 1e-6 on shells 1..2, the force "
-             "field recomputed with hipFFT before every step from the last stored fields; "
-             "fields stored every 10th step), FP32 shifted storage and compute, u0 noise +-1e-3",
+             "amplitude 5e-4 on shells 5..7, energy removal 1e-6 on shells 1..2; the force "
+             "field is a function of the last stored fields and is rebuilt when they change, "
+             "by a direct transform of the 8 removal modes; fields stored every 10th step), "
+             "FP32 shifted storage and compute, u0 noise +-1e-3",
         cfg=dict(lattice="D2Q9", nx=2048, ny=2048, nz=1, tau=0.6,
                  forcing_scheme="ExactDifferenceMethod", force="Turbulent2D",
                  amplitude=(5e-4, 5e-4, 5e-4), force_kmin=5, force_kmax=7,

@@ -109,15 +109,16 @@ typedef struct {
   double residual_tol;  /* absolute H-residual tolerance |G(alpha)| of the
                            LEARNED and WARM_RESIDUAL solvers */
   int kernel;           /* LBM_KERNEL_* */
-  /* Spectral forces (Force.h:294-623), a force field recomputed by the
-   * library with hipFFT; shells are integer |k|^2 in [kmin^2, kmax^2]:
+  /* Spectral forces (Force.h:294-623), a force field built by the library
+   * (hipFFT, or a direct transform for a removal shell of <= 32 modes); shells are integer |k|^2 in [kmin^2, kmax^2]:
    *  CONSTANT_SHELL: F = curl psi, psi_hat = amplitude on the shell
    *    (MakeIncompressible, Transformer.h:301-389), set once;
    *  ENERGY_REMOVAL: F_d = -removal_amplitude_d (rho u_d) restricted to the
    *    removal shell, from the fields of the last storing step (before any:
    *    the initialiser's rho0 and u0, or after lbm_set_populations the
    *    restart fields rho = sum f_i, u = sum c_i f_i, undivided as in
-   *    Initialize.h:124-145), recomputed before every step;
+   *    Initialize.h:124-145), recomputed before a step whenever those fields
+   *    have changed (the result is the same as recomputing every step);
    *  TURBULENT_2D: the sum of both (injection: amplitude, force_kmin..kmax). */
   int force_kmin, force_kmax;     /* forcekMin, forcekMax (Input_prod_2d.in:51-52) */
   double removal_amplitude[3];    /* removalForceAmplitude (Input_prod_2d.in:54) */

@@ -87,6 +87,7 @@ struct lbm_handle {
   double* ffield = nullptr;
   float* ffield32 = nullptr;
   bool ffield_built = false;
+  uint64_t ffield_version = 0;  // fields_version the force field was built from
   // alpha modes 1 and 2: count and list of the cells the correction defers
   unsigned* defer_count = nullptr;
   int* defer_list = nullptr;
@@ -930,10 +931,15 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
                               h->cfg.force == LBM_FORCE_TURBULENT_2D;
   for (int s = 0; s < nsteps; ++s) {
     // Collision::update before the step (Algorithm.h:338, Collision.h:96-99);
-    // ConstantShell is set once (Collision.h:51-54)
-    if (field_force && (time_dependent || !h->ffield_built)) {
+    // ConstantShell is set once (Collision.h:51-54).  EnergyRemoval and
+    // Turbulent2D read only the fields of the last storing step, which change
+    // once per storing step; the update is deterministic, so recomputing it
+    // from unchanged fields would rewrite the same bits and is skipped.
+    if (field_force && (!h->ffield_built ||
+                        (time_dependent && h->ffield_version != h->fields_version))) {
       if (int rc = lbm::update_spectral_force(h)) return rc;
       h->ffield_built = true;
+      h->ffield_version = h->fields_version;
     }
     a.src = h->f[h->cur];
     a.dst = h->f[1 - h->cur];

@@ -918,9 +918,11 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
                 "lbm_mlp_set_normalization");
   if (h->cfg.collision == LBM_ELBM && h->alpha_mode == LBM_ALPHA_LEARNED)
     if (int rc = pack_mlp(h)) return rc;
-  if (h->cfg.collision == LBM_ELBM && h->alpha_mode != LBM_ALPHA_NEWTON && !h->defer_list) {
-    HIP_TRY(h, hipMalloc(&h->defer_count, sizeof(unsigned)));
-    HIP_TRY(h, hipMalloc(&h->defer_list, (size_t)h->N * sizeof(int)));
+  if (h->cfg.collision == LBM_ELBM && h->alpha_mode != LBM_ALPHA_NEWTON) {
+    // each buffer on its own, so a failed second allocation leaks nothing
+    // when the next call retries
+    if (!h->defer_count) HIP_TRY(h, hipMalloc(&h->defer_count, sizeof(unsigned)));
+    if (!h->defer_list) HIP_TRY(h, hipMalloc(&h->defer_list, (size_t)h->N * sizeof(int)));
   }
   StepArgs a = make_args(h);
   // Vector kernel: one wavefront per 64 V-cell segment, 4 per workgroup.
@@ -1069,14 +1071,25 @@ int need_mlp_lattice(lbm_handle* h) {
 int ensure_mlp(lbm_handle* h) {
   if (int rc = need_mlp_lattice(h)) return rc;
   if (!h->mlp) {
-    HIP_TRY(h, hipMalloc(&h->mlp, sizeof(float) * h->mlp_nb));
-    HIP_TRY(h, hipMalloc(&h->mlp_m, sizeof(float) * h->mlp_np));
-    HIP_TRY(h, hipMalloc(&h->mlp_v, sizeof(float) * h->mlp_np));
-    HIP_TRY(h, hipMalloc(&h->mlp_packed, mlp_packed_bytes(h->cfg.lattice)));
+    // all four buffers or none: the handle never holds a partial set
+    float *p = nullptr, *m = nullptr, *v = nullptr;
+    void* packed = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(float) * h->mlp_nb);
+    if (e == hipSuccess) e = hipMalloc(&m, sizeof(float) * h->mlp_np);
+    if (e == hipSuccess) e = hipMalloc(&v, sizeof(float) * h->mlp_np);
+    if (e == hipSuccess) e = hipMalloc(&packed, mlp_packed_bytes(h->cfg.lattice));
+    if (e == hipSuccess) e = hipMemset(p, 0, sizeof(float) * h->mlp_nb);
+    if (e == hipSuccess) e = hipMemset(m, 0, sizeof(float) * h->mlp_np);
+    if (e == hipSuccess) e = hipMemset(v, 0, sizeof(float) * h->mlp_np);
+    if (e != hipSuccess) {
+      for (void* q : {(void*)p, (void*)m, (void*)v, packed}) (void)hipFree(q);
+      return hip_fail(h, e, "network buffers");
+    }
+    h->mlp = p;
+    h->mlp_m = m;
+    h->mlp_v = v;
+    h->mlp_packed = packed;
     h->mlp_dirty = true;
-    HIP_TRY(h, hipMemset(h->mlp, 0, sizeof(float) * h->mlp_nb));
-    HIP_TRY(h, hipMemset(h->mlp_m, 0, sizeof(float) * h->mlp_np));
-    HIP_TRY(h, hipMemset(h->mlp_v, 0, sizeof(float) * h->mlp_np));
   }
   return LBM_OK;
 }
@@ -1263,7 +1276,12 @@ int lbm_sample_log_enable(lbm_handle* h, int64_t capacity, double fraction, uint
     h->log_x = nullptr;
     h->log_alpha = nullptr;
   }
-  if (!h->log_x) {
+  if (!h->log_x || !h->log_alpha) {
+    // a failed earlier attempt may have left one of the pair
+    (void)hipFree(h->log_x);
+    (void)hipFree(h->log_alpha);
+    h->log_x = nullptr;
+    h->log_alpha = nullptr;
     HIP_TRY(h, hipMalloc(&h->log_x, sizeof(float) * (size_t)capacity * h->mlp_nin));
     HIP_TRY(h, hipMalloc(&h->log_alpha, sizeof(double) * (size_t)capacity));
   }

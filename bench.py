@@ -252,12 +252,52 @@ def bytes_per_cell(cfg, wl_cfg) -> int:
     return b
 
 
-def cpu_baseline(wl, max_seconds=15.0):
-    """The FP64 oracle on a bounded sample of the same workload (host cores)."""
+def host_info() -> dict:
+    """CPU model, logical CPUs, the CPUs this process may run on, physical cores."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "cpus_available": len(os.sched_getaffinity(0)),
+            "physical_cores": len(cores) or None,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads() -> int:
+    """Every CPU this process may use: its affinity set, within the host's CPU
+    share when OMP_NUM_THREADS states one."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(wl, max_seconds=10.0):
+    """The FP64 oracle, built -march=native on this host, on a bounded sample
+    of the same workload: one thread (the reference's serial per-rank loop,
+    Computation.h:102-119) and OpenMP over x on every CPU available."""
     import numpy as np
 
     from oracle import oracle as o
-    o.build()
+    o.lib_native()
     c = wl["cfg"]
     names = {"D2Q9": 3, "D3Q19": 9, "D3Q27": 10}
     cfg = o.Config(lattice=names[c["lattice"]],
@@ -270,8 +310,6 @@ def cpu_baseline(wl, max_seconds=15.0):
                    equilibrium=3, nx=c["nx"], ny=c["ny"], nz=c.get("nz", 1), tau=c["tau"],
                    amplitude=c.get("amplitude", (0, 0, 0)),
                    wavelength=c.get("wavelength", (1, 1, 1)))
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    o.lib().ref_set_threads(threads)
     spec = wl["init"]
     if spec[0] == "noise":
         u = o.noise_velocity(cfg, spec[2], spec[1])
@@ -280,33 +318,117 @@ def cpu_baseline(wl, max_seconds=15.0):
     else:
         u = o.taylor_green(cfg, spec[1])
     f = o.init_equilibrium(cfg, np.ones(cfg.cells), u)
-    if cfg.force >= o.FORCE_CONSTANT_SHELL:
-        # spectral force: numpy transforms of the field before every step
-        spec = dict(kmin=c["force_kmin"], kmax=c["force_kmax"],
-                    removal_amplitude=c["removal_amplitude"], removal_kmin=c["removal_kmin"],
-                    removal_kmax=c["removal_kmax"])
-        every = wl.get("store_every", 1)
-        ones = np.ones(cfg.cells)
+    field_force = cfg.force >= o.FORCE_CONSTANT_SHELL
 
-        def steps(k):
-            o.run_field_force(cfg, f, k, ones, u, spec, store_every=every, threads=threads)
-    else:
-        def steps(k):
-            o.run(cfg, f, k, threads=threads)
-    t0 = time.perf_counter()
-    steps(1)
-    t1 = time.perf_counter() - t0
-    n = int(max(1, min(50, (max_seconds - t1) / max(t1, 1e-6))))
-    t0 = time.perf_counter()
-    steps(n)
-    t = time.perf_counter() - t0
-    del u
-    return {"value": round(cfg.cells * n / t / 1e6, 3), "unit": "MLUPS", "cores": threads,
-            "kind": "port",
-            "sample": f"FP64 oracle (OpenMP over x, {threads} threads), {n} steps of the "
-                      f"same {cfg.nx}x{cfg.ny}x{cfg.nz} workload after 1 warm-up step"
-                      + (" (force field by numpy FFT before every step)"
-                         if cfg.force >= o.FORCE_CONSTANT_SHELL else "")}
+    def timed(threads):
+        if field_force:
+            # spectral force: numpy transforms of the field before every step
+            fspec = dict(kmin=c["force_kmin"], kmax=c["force_kmax"],
+                         removal_amplitude=c["removal_amplitude"],
+                         removal_kmin=c["removal_kmin"], removal_kmax=c["removal_kmax"])
+            every = wl.get("store_every", 1)
+            ones = np.ones(cfg.cells)
+            o.lib().ref_set_threads(threads)
+
+            def steps(k):
+                o.run_field_force(cfg, f, k, ones, u, fspec, store_every=every,
+                                  threads=threads)
+        else:
+            def steps(k):
+                o.run(cfg, f, k, threads=threads, native=True)
+        t0 = time.perf_counter()
+        steps(1)
+        t1 = time.perf_counter() - t0
+        n = int(max(1, min(50, (max_seconds - t1) / max(t1, 1e-6))))
+        t0 = time.perf_counter()
+        steps(n)
+        t = time.perf_counter() - t0
+        return round(cfg.cells * n / t / 1e6, 3), n
+
+    threads = cpu_threads()
+    par, npar = timed(threads)
+    ser, nser = timed(1)
+    where = f"same {cfg.nx}x{cfg.ny}x{cfg.nz} workload after 1 warm-up step" + (
+        " (force field by numpy FFT before every step)" if field_force else "")
+    return {"value": par, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle built -march=native, OpenMP over x on {threads} threads, "
+                      f"{npar} steps of the {where}",
+            "serial": {"value": ser, "unit": "MLUPS", "cores": 1,
+                       "sample": f"the same oracle on one thread (the reference's serial "
+                                 f"per-rank loop), {nser} steps of the {where}"},
+            "host": host_info()}
+
+
+def entropic_comparison(which: str, segments: int) -> dict:
+    """Learned alpha vs Newton at equal residual on an entropic workload, all
+    four solvers from the same state.  C4 (D2Q9 2048^2): the first 1000 steps
+    run the reference Newton with the sample log on (the training data), the
+    network is trained on the GPU, then every solver runs steps 1001..5000
+    from that state in `segments` timed segments.  C5 (D3Q27 256^3): 200
+    logged Newton steps, training, then 50 timed steps per solver."""
+    import numpy as np
+
+    import metalbm_amd as m
+    name = {"c4": "d2q9-elbm-2048-learned", "c5": "d3q27-elbm-256-learned"}[which]
+    wl = WORKLOADS[name]
+    learn = wl["learn"]
+    total = 4000 if which == "c4" else 50
+    seg = total // segments
+    cfg = make_config({**wl["cfg"], "alpha_mode": "NEWTON"})
+    out = {"workload": name, "residual_tol": 1e-10, "timed_steps": total,
+           "segment_steps": seg}
+    with m.Algorithm(cfg) as a:
+        init(a, wl["init"])
+        t0 = time.perf_counter()
+        a.sample_log_enable(learn["capacity"], learn["fraction"], 1)
+        a.iterate(learn["log_steps"])
+        a.sample_log_disable()
+        a.synchronize()
+        t1 = time.perf_counter()
+        l0, l1 = a.mlp_train(learn["iters"], learn["batch"], learn["lr"], seed=1)
+        t2 = time.perf_counter()
+        out["training"] = {"logged_samples": len(a.sample_log()[1]),
+                           "log_steps": learn["log_steps"], "log_seconds": round(t1 - t0, 3),
+                           "train_iterations": learn["iters"],
+                           "train_seconds": round(t2 - t1, 3), "loss_first": l0,
+                           "loss_last": l1}
+        f0, al0 = a.populations(), a.alpha()
+        N = cfg.cells
+        for mode in (m.AlphaMode.NEWTON, m.AlphaMode.NEWTON_RESIDUAL,
+                     m.AlphaMode.WARM_RESIDUAL, m.AlphaMode.LEARNED):
+            a.set_populations(f0)
+            a.set_alpha(al0)
+            a.set_alpha_mode(mode, 1e-10)
+            rows, wall = [], 0.0
+            for _ in range(segments):
+                a.reset_elbm_stats()
+                a.synchronize()
+                t0 = time.perf_counter()
+                a.iterate(seg)
+                a.synchronize()
+                t = time.perf_counter() - t0
+                wall += t
+                st = a.elbm_stats()
+                rows.append([round(N * seg / t / 1e6, 1),
+                             round(st["newton_iterations"] / max(1, st["newton"]), 4),
+                             round(st["learned_accepted"] / max(1, st["newton"]), 4),
+                             st["fallback"]])
+            r = np.array(rows, dtype=float)
+            out[mode.name] = {"MLUPS": round(N * seg * segments / wall / 1e6, 1),
+                              "evaluations_per_solved_cell": round(float(r[:, 1].mean()), 4),
+                              "accepted_by_correction": round(float(r[:, 2].mean()), 4),
+                              "fallbacks": int(r[:, 3].sum()),
+                              "segments": rows}
+            log(f"entropic {which} {mode.name}: {out[mode.name]['MLUPS']} MLUPS, "
+                f"{out[mode.name]['evaluations_per_solved_cell']} evaluations per solved cell")
+    nres = out["NEWTON_RESIDUAL"]["MLUPS"]
+    out["learned_vs_newton_residual"] = round(out["LEARNED"]["MLUPS"] / nres, 3)
+    out["learned_vs_reference_newton"] = round(out["LEARNED"]["MLUPS"] / out["NEWTON"]["MLUPS"], 3)
+    out["learned_vs_warm_residual"] = round(out["LEARNED"]["MLUPS"] /
+                                            out["WARM_RESIDUAL"]["MLUPS"], 3)
+    out["segment_columns"] = ["MLUPS", "evaluations_per_solved_cell", "accepted_by_correction",
+                              "fallbacks"]
+    return out
 
 
 def accuracy_study(nsteps: int, every: int, n: int) -> dict:
@@ -536,6 +658,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-entropic", action="store_true",
+                    help="skip the learned-vs-Newton entropic comparison of the default run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 1 GiB (counter calibration)")
     ap.add_argument("--accuracy", action="store_true",
@@ -651,6 +775,10 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": traffic_of(args.workload),
+                     "traffic_source": (f"profiles/traffic_{args.workload}.txt (rocprofv3 --pmc "
+                                        f"FETCH_SIZE and WRITE_SIZE of the same kernel, scaled "
+                                        f"by a copy of known size)"
+                                        if traffic_of(args.workload) else None),
                      "kernel_avg_us": round(kern_s * 1e6, 2)},
     }
     if stats:
@@ -659,6 +787,11 @@ def main(argv=None) -> int:
             stats["newton_iterations"] / max(1, stats["newton"]), 4)
     if learned:
         out["learned_alpha"] = learned
+    if args.workload == DEFAULT and not args.no_entropic:
+        # the metric's second half: entropic MLUPS, learned alpha vs Newton at
+        # equal residual (C4 over its 5000 steps, C5 over 50 steps)
+        out["entropic"] = {"c4": entropic_comparison("c4", 8),
+                           "c5": entropic_comparison("c5", 1)}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(wl)

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 /* Error codes. */
 enum {
@@ -88,8 +88,14 @@ enum {
                              decides (with its alpha = 2 fallback), in a
                              second launch over the cells so deferred.  The
                              alpha state is kept as for NEWTON. */
-  LBM_ALPHA_WARM_RESIDUAL = 2  /* the same with the stored alpha as the start:
+  LBM_ALPHA_WARM_RESIDUAL = 2,  /* the same with the stored alpha as the start:
                              the method without the network */
+  LBM_ALPHA_NEWTON_RESIDUAL = 3  /* plain Newton at the residual tolerance: the
+                             reference Newton (EntropicStep.h:111-140) from the
+                             stored alpha, stopped when |G(alpha)| <=
+                             residual_tol instead of |step| <= 1e-8, at most 50
+                             evaluations, accepted inside (1, alphaMax), else
+                             alpha = 2 (Collision.h:328-349); one launch */
 };
 
 typedef struct {
@@ -107,7 +113,7 @@ typedef struct {
   int compute;          /* LBM_F64 | LBM_F32 */
   int alpha_mode;       /* LBM_ALPHA_* (ELBM) */
   double residual_tol;  /* absolute H-residual tolerance |G(alpha)| of the
-                           LEARNED and WARM_RESIDUAL solvers */
+                           LEARNED, WARM_RESIDUAL and NEWTON_RESIDUAL solvers */
   int kernel;           /* LBM_KERNEL_* */
   /* Spectral forces (Force.h:294-623), a force field built by the library
    * (hipFFT, or a direct transform for a removal shell of <= 32 modes); shells are integer |k|^2 in [kmin^2, kmax^2]:
@@ -159,8 +165,17 @@ const char *lbm_last_error(const lbm_handle *h);
 
 /* Sizes derived from the configuration. */
 int lbm_sizes(const lbm_handle *h, int64_t *cells, int *q, int *d);
-/* The handle's configuration (alpha_mode and residual_tol as last set). */
+/* The handle's configuration (alpha_mode and residual_tol as last set;
+ * kernel: the shape in use, LBM_KERNEL_CELL or LBM_KERNEL_VECTOR). */
 int lbm_get_config(const lbm_handle *h, lbm_config *out);
+
+/* The step kernel lbm_create chooses for cfg without creating anything:
+ * *shape = LBM_KERNEL_CELL or LBM_KERNEL_VECTOR and *vec = the vector
+ * kernel's cells per lane V (0 for the cell kernel).  Host-only (no device
+ * is touched); LBM_E_INVALID for a configuration lbm_create rejects.  No
+ * reference counterpart: the reference has one kernel shape
+ * (Computation.cuh:39-51). */
+int lbm_kernel_shape(const lbm_config *cfg, int *shape, int *vec);
 
 /* f = feq(rho, u) (Initialize.h:106-117).  rho[N], u[D][N] on the host. */
 int lbm_init_equilibrium(lbm_handle *h, const double *rho, const double *u);

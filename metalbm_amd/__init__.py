@@ -31,7 +31,7 @@ from .capi import LBMError, LbmConfigC, LbmElbmStatsC, check, lib
 __all__ = [
     "LatticeType", "CollisionType", "ForcingSchemeType", "ForceType",
     "EquilibriumType", "Precision", "AlphaMode", "KernelShape", "Config", "Algorithm",
-    "LBMError", "lattice_q", "lattice_d",
+    "LBMError", "lattice_q", "lattice_d", "kernel_shape",
 ]
 
 
@@ -84,9 +84,18 @@ class AlphaMode(enum.IntEnum):
     NEWTON = 0          # reference Newton (Collision.h:328-375)
     LEARNED = 1         # MLP start + one certified Newton correction, else NEWTON
     WARM_RESIDUAL = 2   # the same from the stored alpha (no network)
+    NEWTON_RESIDUAL = 3  # plain Newton from the stored alpha until |G| <= residual_tol
 
 
 _QD = {LatticeType.D2Q9: (9, 2), LatticeType.D3Q19: (19, 3), LatticeType.D3Q27: (27, 3)}
+
+
+def kernel_shape(config: "Config"):
+    """(KernelShape, V) that lbm_create would choose for ``config``; host-only."""
+    shape, vec = ctypes.c_int(), ctypes.c_int()
+    c = config.to_c()
+    check(lib().lbm_kernel_shape(ctypes.byref(c), ctypes.byref(shape), ctypes.byref(vec)), None)
+    return KernelShape(shape.value), vec.value
 
 
 def lattice_q(lattice) -> int:

@@ -28,6 +28,7 @@ EXPORTS = (
     "lbm_sample_log_count", "lbm_sample_log_copy", "lbm_mlp_train", "lbm_set_alpha_mode",
     "lbm_get_config", "lbm_write_fields_hdf5", "lbm_write_distribution_hdf5",
     "lbm_read_distribution_hdf5", "lbm_spectral_shells", "lbm_vorticity", "lbm_spectra",
+    "lbm_kernel_shape",
 )
 
 
@@ -124,6 +125,7 @@ def lib() -> ctypes.CDLL:
                                 dp, dp]
     L.lbm_set_alpha_mode.argtypes = [H, ctypes.c_int, ctypes.c_double]
     L.lbm_get_config.argtypes = [H, ctypes.POINTER(LbmConfigC)]
+    L.lbm_kernel_shape.argtypes = [ctypes.POINTER(LbmConfigC), ip, ip]
     cp = ctypes.c_char_p
     L.lbm_write_fields_hdf5.argtypes = [H, cp, cp]
     L.lbm_write_distribution_hdf5.argtypes = [H, cp]

@@ -104,6 +104,10 @@ int fail(lbm_handle* h, int code, const std::string& msg) {
 }
 
 int hip_fail(lbm_handle* h, hipError_t e, const char* what) {
+  // The error is reported through the return code; take it out of HIP's
+  // per-thread last-error state too, so that the launch checks of later calls
+  // (which read hipGetLastError) report only their own launches.
+  (void)hipGetLastError();
   return fail(h, LBM_E_HIP,
               std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -206,8 +210,9 @@ int validate(const lbm_config* c, std::string& why) {
     return LBM_E_INVALID;
   }
   if (c->collision == LBM_ELBM && c->alpha_mode != LBM_ALPHA_NEWTON &&
-      c->alpha_mode != LBM_ALPHA_LEARNED && c->alpha_mode != LBM_ALPHA_WARM_RESIDUAL) {
-    why = "alpha_mode must be NEWTON (0), LEARNED (1) or WARM_RESIDUAL (2)";
+      c->alpha_mode != LBM_ALPHA_LEARNED && c->alpha_mode != LBM_ALPHA_WARM_RESIDUAL &&
+      c->alpha_mode != LBM_ALPHA_NEWTON_RESIDUAL) {
+    why = "alpha_mode must be NEWTON (0), LEARNED (1), WARM_RESIDUAL (2) or NEWTON_RESIDUAL (3)";
     return LBM_E_INVALID;
   }
   if (c->collision == LBM_ELBM && c->alpha_mode == LBM_ALPHA_LEARNED &&
@@ -234,6 +239,66 @@ int pick_block(int nk) {
   for (int b : {256, 192, 128})
     if (nk % b == 0) return b;
   return 256;
+}
+
+// The step kernel for a validated configuration, and in *vec the vector
+// kernel's cells per lane V (0: one cell per lane).  Host-only.
+StepFn choose_kernel(const lbm_config* cfg, int* vec) {
+  *vec = 0;
+  const int ctype = cfg->compute == LBM_F64 ? TYPE_F64 : TYPE_F32;
+  const int D = cfg->lattice == LBM_D2Q9 ? 2 : 3;
+  const int nk = D == 3 ? cfg->nz : cfg->ny;
+  const int elem = cfg->storage == LBM_F64 ? 8 : (cfg->storage == LBM_F32 ? 4 : 2);
+  KernelKey key{cfg->lattice, cfg->collision, src_of(cfg->forcing_scheme),
+                cfg->equilibrium, cfg->storage, ctype, cfg->shifted ? 1 : 0};
+  StepFn step = registry().find(key);
+  // LBM_KERNEL_AUTO takes the vector kernel only where it measured faster
+  // than one cell per lane (MI355X, 100-step runs): D3Q19 with FP32 storage
+  // at 256^3 (+2%) and FP64 storage with FP32 compute (D3Q27 384^3, +2%).
+  // One cell per lane won for D3Q19 FP64 (+12%), D3Q27 FP16 (+10%) and FP32
+  // (+4%) storage, and D2Q9 128^2 FP64 (+48%): more wavefronts in flight.
+  const bool vector_preferred =
+      (cfg->lattice == LBM_D3Q19 && cfg->storage == LBM_F32) ||
+      (cfg->storage == LBM_F64 && cfg->compute == LBM_F32);
+  if (cfg->collision == LBM_BGK &&
+      (cfg->kernel == LBM_KERNEL_VECTOR ||
+       (cfg->kernel == LBM_KERNEL_AUTO && vector_preferred))) {
+    // Vector width V, in this order (measured on D3Q27 384^3 rows):
+    //  1. 16-byte lanes (V sizeof(S) = 16) even if the last 64-lane segment
+    //     of a row is partial (FP32 V=4 split 64 + 32 lanes: +3% over V=2);
+    //  2. else the widest V whose 64-lane segments tile the row (FP16 V=2,
+    //     three full segments: 37.2k MLUPS, V=4 split 64 + 32: 31.8k);
+    //  3. else the widest V dividing the row.
+    // FP16 V=8 only for D2Q9: 27 x 8 (19 x 8) population values alone fill a
+    // wavefront's registers (one wavefront per SIMD, 23.8k MLUPS).
+    static const int cands[3][3] = {{2, 0, 0}, {4, 2, 0}, {8, 4, 2}};
+    static const int cands_f16_3d[3] = {4, 2, 0};
+    const int* cand = (cfg->storage == LBM_F16 && D == 3) ? cands_f16_3d
+                                                           : cands[cfg->storage];
+    auto take = [&](int V) {
+      KernelKey kv = key;
+      kv.vec = V;
+      StepFn fn = registry().find(kv);
+      if (!fn) return false;
+      step = fn;
+      *vec = V;
+      return true;
+    };
+    bool done = false;
+    for (int k = 0; k < 3 && !done; ++k) {
+      const int V = cand[k];
+      done = V != 0 && V * elem == 16 && nk % V == 0 && take(V);
+    }
+    for (int k = 0; k < 3 && !done; ++k) {
+      const int V = cand[k];
+      done = V != 0 && nk % (64 * V) == 0 && take(V);
+    }
+    for (int k = 0; k < 3 && !done; ++k) {
+      const int V = cand[k];
+      done = V != 0 && nk % V == 0 && take(V);
+    }
+  }
+  return step;
 }
 
 // Force tables (Force.h:154-159, 208-215, 262-267), FP64 on the host with the
@@ -331,11 +396,18 @@ int ensure_fields(lbm_handle* h) {
   return LBM_OK;
 }
 
-// The reference's field list holds the initial density and velocity until
-// the first storing step (Initialize.h:20-70, Algorithm.h:150-160); the
-// EnergyRemoval and Turbulent2D forces read them (Force.h:416-431).  drho may
-// be NULL for a uniform rho0; du is the device velocity [D][N].
-hipError_t seed_fields(lbm_handle* h, const double* drho, double rho0, const double* du) {
+// State every initialiser starts from.  The entropic alpha field restarts at
+// 2 (initAlpha, Initialize.h:81-88), so a re-initialised handle matches a
+// fresh one.  The reference's field list holds the initial density and
+// velocity until the first storing step (Initialize.h:20-70,
+// Algorithm.h:150-160); the EnergyRemoval and Turbulent2D forces read them
+// (Force.h:416-431).  drho may be NULL for a uniform rho0; du is the device
+// velocity [D][N].
+hipError_t seed_state(lbm_handle* h, const double* drho, double rho0, const double* du) {
+  if (h->alpha) {
+    const hipError_t e = launch_fill(h->ctype, h->alpha, 2.0, h->N, h->stream);
+    if (e != hipSuccess) return e;
+  }
   if (h->cfg.force != LBM_FORCE_ENERGY_REMOVAL && h->cfg.force != LBM_FORCE_TURBULENT_2D)
     return hipSuccess;
   ++h->fields_version;
@@ -409,6 +481,13 @@ int lbm::record_error(lbm_handle* h, int code, const std::string& msg) {
 
 int lbm::record_hip(lbm_handle* h, hipError_t e, const char* what) {
   return e == hipSuccess ? LBM_OK : hip_fail(h, e, what);
+}
+
+int lbm::reset_alpha(lbm_handle* h) {
+  if (!h->alpha) return LBM_OK;
+  HIP_TRY(h, launch_fill(h->ctype, h->alpha, 2.0, h->N, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return LBM_OK;
 }
 
 int lbm::device_fields(lbm_handle* h, DeviceFields& out) {
@@ -486,6 +565,24 @@ int lbm_get_config(const lbm_handle* h, lbm_config* out) {
   *out = h->cfg;
   out->alpha_mode = h->alpha_mode;
   out->residual_tol = h->residual_tol;
+  out->kernel = h->vec ? LBM_KERNEL_VECTOR : LBM_KERNEL_CELL;
+  return LBM_OK;
+}
+
+int lbm_kernel_shape(const lbm_config* cfg, int* shape, int* vec) {
+  if (!cfg) return fail(nullptr, LBM_E_INVALID, "null argument");
+  std::string why;
+  if (validate(cfg, why) != LBM_OK) return fail(nullptr, LBM_E_INVALID, why);
+  int V = 0;
+  if (!choose_kernel(cfg, &V))
+    return fail(nullptr, LBM_E_INVALID,
+                "no kernel instantiated for this storage/compute combination");
+  if (cfg->kernel == LBM_KERNEL_VECTOR && !V)
+    return fail(nullptr, LBM_E_INVALID,
+                "the vector kernel needs nz (3D) or ny (2D) divisible by V "
+                "(V = 2 for F64, 2 or 4 for F32, 2, 4 or 8 for F16)");
+  if (shape) *shape = V ? LBM_KERNEL_VECTOR : LBM_KERNEL_CELL;
+  if (vec) *vec = V;
   return LBM_OK;
 }
 
@@ -509,58 +606,15 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   h->rows = h->D == 3 ? cfg->nx * cfg->ny : cfg->nx;
   h->block = pick_block(h->nk);
   h->tiles_per_row = (h->nk + h->block - 1) / h->block;
-  KernelKey key{cfg->lattice, cfg->collision, src_of(cfg->forcing_scheme),
-                cfg->equilibrium, cfg->storage, h->ctype, cfg->shifted ? 1 : 0};
-  h->step = registry().find(key);
-  // LBM_KERNEL_AUTO takes the vector kernel only where it measured faster
-  // than one cell per lane (MI355X, 100-step runs): D3Q19 with FP32 storage
-  // at 256^3 (+2%) and FP64 storage with FP32 compute (D3Q27 384^3, +2%).
-  // One cell per lane won for D3Q19 FP64 (+12%), D3Q27 FP16 (+10%) and FP32
-  // (+4%) storage, and D2Q9 128^2 FP64 (+48%): more wavefronts in flight.
-  const bool vector_preferred =
-      (cfg->lattice == LBM_D3Q19 && cfg->storage == LBM_F32) ||
-      (cfg->storage == LBM_F64 && cfg->compute == LBM_F32);
-  if (cfg->collision == LBM_BGK &&
-      (cfg->kernel == LBM_KERNEL_VECTOR ||
-       (cfg->kernel == LBM_KERNEL_AUTO && vector_preferred))) {
-    // Vector width V, in this order (measured on D3Q27 384^3 rows):
-    //  1. 16-byte lanes (V sizeof(S) = 16) even if the last 64-lane segment
-    //     of a row is partial (FP32 V=4 split 64 + 32 lanes: +3% over V=2);
-    //  2. else the widest V whose 64-lane segments tile the row (FP16 V=2,
-    //     three full segments: 37.2k MLUPS, V=4 split 64 + 32: 31.8k);
-    //  3. else the widest V dividing the row.
-    // FP16 V=8 only for D2Q9: 27 x 8 (19 x 8) population values alone fill a
-    // wavefront's registers (one wavefront per SIMD, 23.8k MLUPS).
-    static const int cands[3][3] = {{2, 0, 0}, {4, 2, 0}, {8, 4, 2}};
-    static const int cands_f16_3d[3] = {4, 2, 0};
-    const int* cand = (cfg->storage == LBM_F16 && h->D == 3) ? cands_f16_3d
-                                                              : cands[cfg->storage];
-    auto take = [&](int V) {
-      KernelKey kv = key;
-      kv.vec = V;
-      StepFn fn = registry().find(kv);
-      if (!fn) return false;
-      h->step = fn;
-      h->vec = V;
-      h->block = 256;
-      h->tiles_per_row = (h->nk + 64 * V - 1) / (64 * V);
-      return true;
-    };
-    bool done = false;
-    for (int k = 0; k < 3 && !done; ++k) {
-      const int V = cand[k];
-      done = V != 0 && V * h->elem == 16 && h->nk % V == 0 && take(V);
-    }
-    for (int k = 0; k < 3 && !done; ++k) {
-      const int V = cand[k];
-      done = V != 0 && h->nk % (64 * V) == 0 && take(V);
-    }
-    for (int k = 0; k < 3 && !done; ++k) {
-      const int V = cand[k];
-      done = V != 0 && h->nk % V == 0 && take(V);
-    }
+  int V = 0;
+  h->step = choose_kernel(cfg, &V);
+  if (V) {
+    h->vec = V;
+    h->block = 256;
+    h->tiles_per_row = (h->nk + 64 * V - 1) / (64 * V);
   }
   auto bail = [&](int code) {
+    (void)hipGetLastError();  // reported here; see hip_fail
     g_create_error = h->err;
     release(h);
     delete h;
@@ -634,7 +688,7 @@ int lbm_init_equilibrium(lbm_handle* h, const double* rho, const double* u) {
   HIP_TRY(h, launch_init_equilibrium(h->cfg.lattice, h->cfg.equilibrium,
                                      h->cfg.storage, h->cfg.shifted,
                                      h->f[h->cur], drho, 0.0, du, h->N, h->stream));
-  HIP_TRY(h, seed_fields(h, drho, 0.0, du));
+  HIP_TRY(h, seed_state(h, drho, 0.0, du));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return LBM_OK;
 }
@@ -648,7 +702,7 @@ int lbm_init_noise(lbm_handle* h, double rho0, double amp, uint64_t seed) {
                                      h->cfg.storage, h->cfg.shifted,
                                      h->f[h->cur], nullptr, rho0, du, h->N,
                                      h->stream));
-  HIP_TRY(h, seed_fields(h, nullptr, rho0, du));
+  HIP_TRY(h, seed_state(h, nullptr, rho0, du));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   return LBM_OK;
 }
@@ -679,7 +733,7 @@ int lbm_init_taylor_green(lbm_handle* h, double rho0, double u0) {
   if (e == hipSuccess)
     e = launch_init_equilibrium(c.lattice, c.equilibrium, c.storage, c.shifted,
                                 h->f[h->cur], nullptr, rho0, du, h->N, h->stream);
-  if (e == hipSuccess) e = seed_fields(h, nullptr, rho0, du);
+  if (e == hipSuccess) e = seed_state(h, nullptr, rho0, du);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(tab);
   if (e != hipSuccess) return hip_fail(h, e, "Taylor-Green initialisation");
@@ -835,7 +889,7 @@ int lbm_init_spectral(lbm_handle* h, double rho0, double urms, int kmax,
   if (e == hipSuccess)
     e = launch_init_equilibrium(c.lattice, c.equilibrium, c.storage, c.shifted,
                                 h->f[h->cur], nullptr, rho0, du, h->N, h->stream);
-  if (e == hipSuccess) e = seed_fields(h, nullptr, rho0, du);
+  if (e == hipSuccess) e = seed_state(h, nullptr, rho0, du);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(buf);
   if (e != hipSuccess) return hip_fail(h, e, "spectral initialisation");
@@ -918,7 +972,8 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
                 "lbm_mlp_set_normalization");
   if (h->cfg.collision == LBM_ELBM && h->alpha_mode == LBM_ALPHA_LEARNED)
     if (int rc = pack_mlp(h)) return rc;
-  if (h->cfg.collision == LBM_ELBM && h->alpha_mode != LBM_ALPHA_NEWTON) {
+  if (h->cfg.collision == LBM_ELBM &&
+      (h->alpha_mode == LBM_ALPHA_LEARNED || h->alpha_mode == LBM_ALPHA_WARM_RESIDUAL)) {
     // each buffer on its own, so a failed second allocation leaks nothing
     // when the next call retries
     if (!h->defer_count) HIP_TRY(h, hipMalloc(&h->defer_count, sizeof(unsigned)));
@@ -1270,14 +1325,14 @@ int lbm_sample_log_enable(lbm_handle* h, int64_t capacity, double fraction, uint
   if (int rc = need_mlp_lattice(h)) return rc;
   if (capacity <= 0 || !(fraction > 0.0) || fraction > 1.0)
     return fail(h, LBM_E_INVALID, "capacity > 0 and 0 < fraction <= 1 required");
-  if (h->log_x && capacity != h->log_cap) {
-    (void)hipFree(h->log_x);
-    (void)hipFree(h->log_alpha);
-    h->log_x = nullptr;
-    h->log_alpha = nullptr;
-  }
-  if (!h->log_x || !h->log_alpha) {
-    // a failed earlier attempt may have left one of the pair
+  // Logging stays off until every buffer of the new capacity exists, so a
+  // failed allocation below leaves the steps logging nothing (make_args hands
+  // the kernels log buffers only while log_on).
+  h->log_on = false;
+  const bool resize = h->log_cap != capacity;
+  h->log_cap = 0;
+  if (resize || !h->log_x || !h->log_alpha) {
+    // also frees what a failed earlier attempt may have left of the pair
     (void)hipFree(h->log_x);
     (void)hipFree(h->log_alpha);
     h->log_x = nullptr;
@@ -1286,7 +1341,7 @@ int lbm_sample_log_enable(lbm_handle* h, int64_t capacity, double fraction, uint
     HIP_TRY(h, hipMalloc(&h->log_alpha, sizeof(double) * (size_t)capacity));
   }
   if (!h->log_count) HIP_TRY(h, hipMalloc(&h->log_count, sizeof(unsigned long long)));
-  HIP_TRY(h, hipMemsetAsync(h->log_count, 0, sizeof(unsigned long long), h->stream));
+  HIP_TRY(h, launch_fill(TYPE_F64, h->log_count, 0.0, 1, h->stream));
   h->log_cap = capacity;
   h->log_threshold = fraction >= 1.0 ? 0xFFFFFFFFu : (unsigned)(fraction * 4294967296.0);
   h->log_salt = salt;
@@ -1448,8 +1503,10 @@ int lbm_mlp_train(lbm_handle* h, int iterations, int batch, double lr, uint64_t 
 int lbm_set_alpha_mode(lbm_handle* h, int mode, double residual_tol) {
   if (!h) return LBM_E_INVALID;
   if (h->cfg.collision != LBM_ELBM) return fail(h, LBM_E_STATE, "alpha mode belongs to ELBM");
-  if (mode != LBM_ALPHA_NEWTON && mode != LBM_ALPHA_LEARNED && mode != LBM_ALPHA_WARM_RESIDUAL)
-    return fail(h, LBM_E_INVALID, "alpha_mode must be NEWTON (0), LEARNED (1) or WARM_RESIDUAL (2)");
+  if (mode != LBM_ALPHA_NEWTON && mode != LBM_ALPHA_LEARNED && mode != LBM_ALPHA_WARM_RESIDUAL &&
+      mode != LBM_ALPHA_NEWTON_RESIDUAL)
+    return fail(h, LBM_E_INVALID, "alpha_mode must be NEWTON (0), LEARNED (1), WARM_RESIDUAL (2) "
+                                  "or NEWTON_RESIDUAL (3)");
   if (mode == LBM_ALPHA_LEARNED) {
     if (int rc = need_mlp_lattice(h)) return rc;
   }

@@ -244,8 +244,9 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 
 // Entropic step, one instantiation per alpha solver MODE (0: reference
 // Newton; 1: network start, 2: stored-alpha start, each with one certified
-// correction and the reference Newton as fallback) so that no kernel carries
-// the registers of another's solver.
+// correction and the reference Newton as fallback; 3: that fallback over the
+// deferred cells; 4: plain Newton from the stored alpha to the residual
+// tolerance) so that no kernel carries the registers of another's solver.
 // Three phases so that the learned mode's wavefront-wide matrix-core
 // inference runs with every lane present: (1) pull, moments, fneq and the
 // alpha prelude per cell; (2) MODE 1 only: the network predicts alpha for
@@ -363,6 +364,8 @@ __global__ void __launch_bounds__(BLOCK)
       const Fp64View<L, C, SH> p{f, fneq};
       if constexpr (MODE == 0 || MODE == 3) {
         alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, its);
+      } else if constexpr (MODE == 4) {
+        alpha = newton_residual<L>(p, amax, double(astate[cell]), a.residual_tol, branch, its);
       } else {
         // MODE 1 starts from the network's alpha, MODE 2 from the stored one;
         // one certified correction, else the reference Newton decides -- in
@@ -378,10 +381,10 @@ __global__ void __launch_bounds__(BLOCK)
         }
         its = ev;
       }
-      // The sample log records the reference Newton's solves: every solve in
-      // MODE 0, the solves the correction deferred in MODE 3 (the cells the
+      // The sample log records the Newton solves: every solve in MODE 0 and
+      // MODE 4, the solves the correction deferred in MODE 3 (the cells the
       // network or the stored alpha missed -- data to retrain on).
-      if constexpr (Shape::NIN > 0 && (MODE == 0 || MODE == 3)) {
+      if constexpr (Shape::NIN > 0 && (MODE == 0 || MODE == 3 || MODE == 4)) {
         if (a.log_cap > 0 && branch == 2 &&
             sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {
           const unsigned long long k = atomicAdd(a.log_count, 1ull);

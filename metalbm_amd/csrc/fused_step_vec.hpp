@@ -68,10 +68,14 @@ struct WavePos {
   bool whole_row;
 };
 
-// Buffer-resource access (MUBUF): one SGPR resource per population lattice
-// row block, the row offset in soffset (uniform) and a single 32-bit lane
+// Buffer-resource access (MUBUF): one SGPR resource per population row, its
+// 64-bit base formed in scalar registers (row start = lattice + i N + row
+// offset) and its extent the row's nk values, plus a single 32-bit lane
 // offset shared by all Q loads and stores, so no per-lane 64-bit address
-// arithmetic is needed.  Accesses beyond num_records return 0 / are dropped.
+// arithmetic is needed and no 32-bit byte count or offset depends on the
+// lattice size (a resource spanning a whole population would overflow its
+// 32-bit num_records from 2^32 bytes on, e.g. D3Q19 FP32 at 1024^3).
+// Accesses beyond num_records return 0 / are dropped.
 constexpr int kBufferDword3 = 0x00020000;  // raw buffer, 32-bit data format
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base,
@@ -129,11 +133,11 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
   } else {
     row_off = xs * a.ny;
   }
-  const S* pop = static_cast<const S*>(a.src) + (size_t)i * (size_t)a.N;
-  const __amdgpu_buffer_rsrc_t r = make_rsrc(pop, (unsigned)a.N * (unsigned)sizeof(S));
+  const S* row = static_cast<const S*>(a.src) + (size_t)i * (size_t)a.N + (size_t)row_off;
+  const __amdgpu_buffer_rsrc_t r = make_rsrc(row, (unsigned)a.nk * (unsigned)sizeof(S));
   using vt = typename VecT<S, V>::type;
   using IO = BufIO<V * (int)sizeof(S)>;
-  const vt v = __builtin_bit_cast(vt, IO::load(r, voff, row_off * (int)sizeof(S)));
+  const vt v = __builtin_bit_cast(vt, IO::load(r, voff, 0));
   S s[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) s[j] = v[j];
@@ -146,7 +150,7 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
     S edge = lane_shift<true>(w.xfer, s[V - 1]);
     if (!w.whole_row) {
       const int kb = w.seg_lo == 0 ? a.nk - 1 : w.seg_lo - 1;
-      const S b = pop[(unsigned)(row_off + kb)];
+      const S b = row[kb];
       if (w.lane == 0) edge = b;
     }
     out[0] = edge;
@@ -157,7 +161,7 @@ __device__ __forceinline__ void pull_vec(const StepArgs& a, const WavePos& w,
     S edge = lane_shift<false>(w.xfer, s[0]);
     if (!w.whole_row) {
       const int kb = w.seg_hi == a.nk ? 0 : w.seg_hi;
-      const S b = pop[(unsigned)(row_off + kb)];
+      const S b = row[kb];
       if (w.lane == w.last) edge = b;
     }
 #pragma unroll
@@ -185,16 +189,14 @@ __device__ __forceinline__ void store_vec(const StepArgs& a, const WavePos& w,
     else
       o[j] = S(v[j]);
   }
-  S* pop = static_cast<S*>(a.dst) + (size_t)i * (size_t)a.N;
-  // The row offset rides in the lane offset and soffset is the constant 0.
-  // With a register soffset the compiler assumes a >8-byte store cannot be
-  // corrupted by a VALU write to its data VGPRs right after issue, and emits
-  // none of the wait states it otherwise inserts; on gfx950 that overwrote
-  // store data at full size.
+  S* row = static_cast<S*>(a.dst) + (size_t)i * (size_t)a.N + (size_t)self_row;
+  // soffset is the constant 0.  With a register soffset the compiler assumes
+  // a >8-byte store cannot be corrupted by a VALU write to its data VGPRs
+  // right after issue, and emits none of the wait states it otherwise
+  // inserts; on gfx950 that overwrote store data at full size.
   if (w.active)
     IO::store(__builtin_bit_cast(typename IO::raw, o),
-              make_rsrc(pop, (unsigned)a.N * (unsigned)sizeof(S)),
-              voff + (unsigned)self_row * (unsigned)sizeof(S), 0);
+              make_rsrc(row, (unsigned)a.nk * (unsigned)sizeof(S)), voff, 0);
 }
 
 template <class L, class S, class C, int FS, int EQ, bool SH, int V>

@@ -295,7 +295,10 @@ int lbm_read_distribution_hdf5(lbm_handle* h, const char* path) {
     if (!read_dataset(x, file.id(), g, "distribution" + std::to_string(i),
                       f.data() + (size_t)i * g.N, why))
       return lbm::record_error(h, LBM_E_INVALID, why);
-  return lbm_set_populations(h, f.data());
+  if (int rc = lbm_set_populations(h, f.data())) return rc;
+  // a restart starts the entropic solve from alpha = 2 as a fresh run does
+  // (initAlpha, Initialize.h:81-88)
+  return lbm::reset_alpha(h);
 }
 
 }  // extern "C"

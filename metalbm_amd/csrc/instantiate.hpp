@@ -19,6 +19,10 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
   else if (a.alpha_mode == 0)
     hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 0>), dim3(grid),
                        dim3(block), 0, st, a, fsx);
+  else if (a.alpha_mode == 3)
+    // plain Newton from the stored alpha to the residual tolerance (MODE 4)
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 4>), dim3(grid),
+                       dim3(block), 0, st, a, fsx);
   else {
     // Modes 1 and 2: the certified correction, then the reference Newton for
     // the cells it deferred (MODE 3, one thread per possible list entry).

@@ -433,4 +433,59 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
   return true;
 }
 
+// Plain Newton to the residual tolerance: from `start` (the stored alpha),
+// alpha <- alpha - G/G' until |G(alpha)| <= tol, at most 50 evaluations, G
+// in the same log1p form as the certified correction.  Accepted only inside
+// the reference's bounds (1, amax); otherwise -- no convergence, a step
+// leaving the logarithm's domain (f - alpha fneq <= 0 for some i), or a root
+// outside the bounds -- the reference's fallback alpha = 2
+// (Collision.h:328-349) applies.  This is the reference Newton
+// (EntropicStep.h:111-140) with the H-residual as its stopping rule instead of
+// |step| <= 1e-8.  branch: 2 solved, 3 fallback; evals: G evaluations.
+template <class L, class V>
+__device__ __forceinline__ double newton_residual(const V& p, double amax, double start,
+                                                  double tol, int& branch, int& evals) {
+  constexpr int Q = L::dimQ;
+  evals = 0;
+  branch = 3;
+  const XTerms<L, V> X(p);
+  double ymax = 0.0;
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    ymax = fmax(ymax, fabs(fma(p.template f<i>(), 1.0 / L::W[i], -1.0)));
+  });
+  double S = 0.0, Sn = 0.0;  // sum n ln(f/w), sum n
+  auto sums = [&](auto log1p_of) {
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const double ni = p.template n<i>();
+      S += ni * log1p_of(fma(p.template f<i>(), 1.0 / L::W[i], -1.0));
+      Sn += ni;
+    });
+  };
+  if (ymax <= kSeriesMax)
+    sums([](double y) { return log1p_small(y); });
+  else if (ymax <= kMidMax)
+    sums([](double y) { return log1p_mid(y); });
+  else
+    sums([](double y) { return log1p(y); });
+  double a = start;
+  for (int it = 1; it <= 50; ++it) {
+    // the logarithm needs f_i - a fneq_i > 0 for every i: 0 <= a max x_i < 1
+    if (!(a >= 0.0 && a * X.xmax < 1.0)) break;
+    double g, dg, h;
+    residual_terms<L>(p, X, S, Sn, a, g, dg, h);
+    evals = it;
+    if (fabs(g) <= tol) {
+      if (a > 1.0 && a < amax) {
+        branch = 2;
+        return a;
+      }
+      break;
+    }
+    a -= g / dg;
+  }
+  return 2.0;
+}
+
 }  // namespace lbm

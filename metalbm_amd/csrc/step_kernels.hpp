@@ -50,7 +50,8 @@ struct StepArgs {
   unsigned long long* stats;  // 6 ELBM counters or NULL
   // Entropic alpha solver: 0 reference Newton; 1 network start, 2 stored
   // alpha start, each with one certified correction (|G| <= residual_tol or
-  // Taylor bound), else the reference Newton.
+  // Taylor bound), else the reference Newton; 3 plain Newton from the stored
+  // alpha until |G| <= residual_tol.
   int alpha_mode;
   double residual_tol;
   const void* mlp_packed;  // network in its LDS layout (alpha mode 1)

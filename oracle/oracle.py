@@ -51,6 +51,8 @@ def build() -> str:
 
 
 _lib = None
+_lib_native = None
+_NATIVE_PATH = os.path.join(_HERE, "_native", "libcpu_ref.so")
 
 
 def lib():
@@ -58,53 +60,66 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        dp = ctypes.POINTER(ctypes.c_double)
-        cp = ctypes.POINTER(_Config)
-        L.ref_lattice_q.argtypes = [ctypes.c_int]
-        L.ref_lattice_d.argtypes = [ctypes.c_int]
-        L.ref_lattice_c.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        L.ref_lattice_w.argtypes = [ctypes.c_int, ctypes.c_int]
-        L.ref_lattice_w.restype = ctypes.c_double
-        L.ref_equilibrium.argtypes = [cp, ctypes.c_double, dp, ctypes.c_int]
-        L.ref_equilibrium.restype = ctypes.c_double
-        L.ref_force_at.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
-        L.ref_init_equilibrium.argtypes = [cp, dp, dp, dp]
-        L.ref_step.argtypes = [cp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(_Stats)]
-        L.ref_step_field.argtypes = [cp, dp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(_Stats)]
-        L.ref_run.argtypes = [cp, dp, dp, dp, ctypes.c_int, dp, dp, dp,
-                              ctypes.POINTER(_Stats)]
-        L.ref_elbm_alpha.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double,
-                                     ctypes.POINTER(ctypes.c_int),
-                                     ctypes.POINTER(ctypes.c_int)]
-        L.ref_elbm_alpha.restype = ctypes.c_double
-        L.ref_entropic_g.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
-        L.ref_entropic_g.restype = ctypes.c_double
-        L.ref_entropic_dg.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
-        L.ref_entropic_dg.restype = ctypes.c_double
-        L.ref_splitmix64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
-        L.ref_splitmix64.restype = ctypes.c_uint64
-        L.ref_init_noise_velocity.argtypes = [cp, ctypes.c_uint64, ctypes.c_double, dp]
-        L.ref_init_taylor_green.argtypes = [cp, ctypes.c_double, dp]
-        L.ref_init_spectral_velocity.argtypes = [cp, ctypes.c_double, ctypes.c_int,
-                                                 ctypes.c_double, ctypes.c_uint64, dp]
-        L.ref_init_spectral_velocity.restype = ctypes.c_int
-        L.ref_reduce.argtypes = [cp, dp, dp, dp, dp, dp]
-        L.ref_set_threads.argtypes = [ctypes.c_int]
-        L.mlp_ref_param_count.argtypes = [ctypes.c_int] * 3
-        L.mlp_ref_forward.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                      ctypes.c_int, dp, dp]
-        L.mlp_ref_loss.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, dp, dp]
-        L.mlp_ref_loss.restype = ctypes.c_double
-        L.mlp_ref_grad.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                   ctypes.c_int, dp, dp, dp]
-        L.mlp_ref_grad.restype = ctypes.c_double
-        L.mlp_ref_adam.argtypes = [dp, dp, dp, dp, ctypes.c_int, ctypes.c_double,
-                                   ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                   ctypes.c_int]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
     return _lib
+
+
+def lib_native():
+    """The same oracle compiled -march=native on this host (make native):
+    the timed CPU baseline, built where it is timed."""
+    global _lib_native
+    if _lib_native is None:
+        subprocess.run(["make", "-s", "-C", _HERE, "native"], check=True)
+        _lib_native = _bind(ctypes.CDLL(_NATIVE_PATH))
+    return _lib_native
+
+
+def _bind(L):
+    dp = ctypes.POINTER(ctypes.c_double)
+    cp = ctypes.POINTER(_Config)
+    L.ref_lattice_q.argtypes = [ctypes.c_int]
+    L.ref_lattice_d.argtypes = [ctypes.c_int]
+    L.ref_lattice_c.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.ref_lattice_w.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.ref_lattice_w.restype = ctypes.c_double
+    L.ref_equilibrium.argtypes = [cp, ctypes.c_double, dp, ctypes.c_int]
+    L.ref_equilibrium.restype = ctypes.c_double
+    L.ref_force_at.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+    L.ref_init_equilibrium.argtypes = [cp, dp, dp, dp]
+    L.ref_step.argtypes = [cp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(_Stats)]
+    L.ref_step_field.argtypes = [cp, dp, dp, dp, dp, dp, dp, dp, ctypes.POINTER(_Stats)]
+    L.ref_run.argtypes = [cp, dp, dp, dp, ctypes.c_int, dp, dp, dp,
+                          ctypes.POINTER(_Stats)]
+    L.ref_elbm_alpha.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double,
+                                 ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(ctypes.c_int)]
+    L.ref_elbm_alpha.restype = ctypes.c_double
+    L.ref_entropic_g.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
+    L.ref_entropic_g.restype = ctypes.c_double
+    L.ref_entropic_dg.argtypes = [ctypes.c_int, dp, dp, ctypes.c_double]
+    L.ref_entropic_dg.restype = ctypes.c_double
+    L.ref_splitmix64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L.ref_splitmix64.restype = ctypes.c_uint64
+    L.ref_init_noise_velocity.argtypes = [cp, ctypes.c_uint64, ctypes.c_double, dp]
+    L.ref_init_taylor_green.argtypes = [cp, ctypes.c_double, dp]
+    L.ref_init_spectral_velocity.argtypes = [cp, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_uint64, dp]
+    L.ref_init_spectral_velocity.restype = ctypes.c_int
+    L.ref_reduce.argtypes = [cp, dp, dp, dp, dp, dp]
+    L.ref_set_threads.argtypes = [ctypes.c_int]
+    L.mlp_ref_param_count.argtypes = [ctypes.c_int] * 3
+    L.mlp_ref_forward.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, dp, dp]
+    L.mlp_ref_loss.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, dp, dp]
+    L.mlp_ref_loss.restype = ctypes.c_double
+    L.mlp_ref_grad.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, dp, dp, dp]
+    L.mlp_ref_grad.restype = ctypes.c_double
+    L.mlp_ref_adam.argtypes = [dp, dp, dp, dp, ctypes.c_int, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_int]
+    return L
 
 
 def _p(a):
@@ -227,8 +242,9 @@ class RunResult:
 
 
 def run(cfg: Config, f0: np.ndarray, nsteps: int, alpha0: np.ndarray | None = None,
-        threads: int = 1) -> RunResult:
-    """Step the oracle nsteps times from f0; fields are those of the last step."""
+        threads: int = 1, native: bool = False) -> RunResult:
+    """Step the oracle nsteps times from f0; fields are those of the last step.
+    native: run the -march=native build of the same sources (timing only)."""
     N, Q, D = cfg.cells, cfg.q, cfg.d
     f = np.array(f0, dtype=np.float64, order="C").reshape(Q, N).copy()
     scratch = np.empty_like(f)
@@ -238,7 +254,7 @@ def run(cfg: Config, f0: np.ndarray, nsteps: int, alpha0: np.ndarray | None = No
     u = np.zeros((D, N))
     F = np.zeros((D, N))
     st = _Stats()
-    L = lib()
+    L = lib_native() if native else lib()
     L.ref_set_threads(threads)
     k = cfg.c()
     rc = L.ref_run(ctypes.byref(k), _p(f), _p(scratch), _p(alpha), nsteps,

@@ -35,7 +35,7 @@ def test_every_declared_symbol_is_exported(capi):
 
 def test_abi_version_and_defaults(capi):
     L = capi.lib()
-    assert L.lbm_abi_version() == 2
+    assert L.lbm_abi_version() == 3
     c = capi.LbmConfigC()
     assert L.lbm_config_default(ctypes.byref(c)) == 0
     assert (c.lattice, c.collision, c.forcing_scheme, c.equilibrium) == (3, 1, 2, 3)
@@ -87,3 +87,44 @@ def test_null_handle_calls_fail_cleanly(capi):
     assert L.lbm_write_fields_hdf5(None, b"x.h5", None) == capi.LBM_E_INVALID
     assert L.lbm_write_distribution_hdf5(None, b"x.h5") == capi.LBM_E_INVALID
     assert L.lbm_read_distribution_hdf5(None, b"x.h5") == capi.LBM_E_INVALID
+
+
+# (lattice, extents, storage, compute, shifted, kernel request) -> (shape, V)
+# as lbm_create chooses them; 1 = one cell per lane, 2 = vector kernel.
+SHAPES = [
+    ((9, (256, 256, 256), 1, 1, 1, 0), (2, 4)),     # headline D3Q19 FP32: V = 4
+    ((9, (1024, 1024, 1024), 1, 1, 1, 0), (2, 4)),  # 2^32 bytes per population array
+    ((9, (1024, 1024, 512), 0, 1, 1, 0), (2, 2)),   # FP64 storage, FP32 compute, 2^32 B
+    ((9, (256, 256, 256), 0, 0, 0, 0), (1, 0)),     # FP64 reference order: cell kernel
+    ((10, (384, 384, 384), 2, 1, 1, 0), (1, 0)),    # D3Q27 FP16: cell kernel
+    ((10, (384, 384, 384), 2, 1, 1, 2), (2, 2)),    # FP16 asked for: V = 2, full segments
+    ((10, (384, 384, 384), 1, 1, 1, 2), (2, 4)),    # FP32 asked for: 16-byte lanes
+    ((3, (128, 128, 1), 0, 0, 0, 0), (1, 0)),       # C1 parity config
+    ((3, (2048, 2048, 1), 2, 1, 1, 2), (2, 8)),     # 2D FP16: V = 8
+    ((9, (4, 4, 100), 0, 0, 0, 2), (2, 2)),         # partial segment: 50 of 64 lanes
+    ((9, (4, 4, 99), 0, 0, 0, 1), (1, 0)),          # cell kernel asked for
+]
+
+
+@pytest.mark.parametrize("case,expect", SHAPES)
+def test_kernel_shape_selection(capi, case, expect):
+    """The kernel shape and cells per lane lbm_create would choose, resolved
+    on the host (lbm_kernel_shape) -- including grids whose population arrays
+    hold 2^32 bytes or more, which the vector kernel addresses through
+    per-row buffer bases."""
+    from metalbm_amd import Config, kernel_shape
+    lat, (nx, ny, nz), storage, compute, shifted, kernel = case
+    cfg = Config(lattice=lat, nx=nx, ny=ny, nz=nz, storage=storage, compute=compute,
+                 shifted=bool(shifted), kernel=kernel)
+    shape, vec = kernel_shape(cfg)
+    assert (int(shape), vec) == expect
+
+
+def test_kernel_shape_rejects_what_create_rejects(capi):
+    from metalbm_amd import Config, LBMError, kernel_shape
+    with pytest.raises(LBMError, match="divisible by V"):
+        kernel_shape(Config(lattice=9, nx=2, ny=2, nz=99, kernel=2))
+    with pytest.raises(LBMError, match="2\\^31"):
+        kernel_shape(Config(lattice=9, nx=2048, ny=2048, nz=1024))
+    with pytest.raises(LBMError, match="BGK only"):
+        kernel_shape(Config(lattice=9, nx=4, ny=4, nz=4, collision=2, kernel=2))

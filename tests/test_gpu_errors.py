@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from metalbm_amd import (AlphaMode, Algorithm, CollisionType, Config, LatticeType, LBMError)
-from metalbm_amd.capi import LBM_E_INVALID, LBM_E_STATE
+from metalbm_amd.capi import LBM_E_HIP, LBM_E_INVALID, LBM_E_NOMEM, LBM_E_STATE
 
 pytestmark = pytest.mark.gpu
 
@@ -66,3 +66,28 @@ def test_bad_arguments_are_rejected():
         expect(LBM_E_INVALID, "bad samples", a.mlp_predict, np.zeros((0, 9), np.float32))
         a.init_spectral(1.0, 0.01, 3, 2.0)  # the handle still works
         a.iterate(1)
+
+
+def test_failed_sample_log_enable_logs_nothing():
+    """A sample-log enable whose device allocation fails returns an error and
+    leaves logging off: the following Newton steps log nothing and the handle
+    keeps working; a later feasible enable logs again."""
+    print("case errors failed sample-log enable", flush=True)
+    cfg = Config(lattice=LatticeType.D2Q9, collision=CollisionType.ELBM, nx=32, ny=32,
+                 tau=0.5003)
+    with Algorithm(cfg) as a:
+        a.init_noise(1.0, 3e-2, 7)
+        a.sample_log_enable(4096, 1.0, 1)
+        a.iterate(2)
+        assert len(a.sample_log()[1]) > 0
+        # 2^40 samples of 9 FP32 features: 36 TB, beyond any device
+        with pytest.raises(LBMError) as e:
+            a.sample_log_enable(2 ** 40, 1.0, 1)
+        assert e.value.code in (LBM_E_HIP, LBM_E_NOMEM), e.value
+        a.iterate(2)
+        assert len(a.sample_log()[1]) == 0
+        st = a.elbm_stats()
+        assert st["newton"] > 0   # the steps did solve cells, and logged none
+        a.sample_log_enable(4096, 1.0, 1)
+        a.iterate(1)
+        assert len(a.sample_log()[1]) > 0

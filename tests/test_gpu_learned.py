@@ -10,8 +10,12 @@
   carries the oracle's reference-Newton alpha from the same stored alpha
   (the fallback).
 * Ablation: the same certified correction started from the stored alpha
-  (the method without the network) on the same state; the test reports what
-  the network changes in evaluations per solved cell.
+  (the method without the network), and plain Newton to the same residual
+  tolerance, on the same state; the test reports what the network changes in
+  evaluations per solved cell and asserts a strict gain from the alpha = 2
+  state, where the stored alpha is a poor start.
+* The C4 configuration in its FP32 build (the one its throughput is timed
+  in), every alpha solver, against the oracle.
 """
 import sys
 
@@ -84,6 +88,10 @@ def test_mlp_gradient_and_forward_match_oracle(oracle, lat):
     assert np.all(err <= bound), (err / bound).max()
 
 
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
 def pulled_state(o, oc, f):
     """Pulled populations and fneq = f - feq(rho, u) per cell (oracle order)."""
     c = o.velocities(oc.lattice)
@@ -138,13 +146,24 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
         f0 = a.populations()
         alpha0 = a.alpha()
         res = {}
-        for mode in (AlphaMode.NEWTON, AlphaMode.WARM_RESIDUAL, AlphaMode.LEARNED):
+        for mode in (AlphaMode.NEWTON, AlphaMode.WARM_RESIDUAL, AlphaMode.LEARNED,
+                     AlphaMode.NEWTON_RESIDUAL):
             a.set_populations(f0)
             a.set_alpha(alpha0)
             a.reset_elbm_stats()
             a.set_alpha_mode(mode, tol)
             a.iterate(1, store=True)
             res[mode] = (a.store_fields()[2], a.elbm_stats(), a.populations())
+        # the same step from the alpha = 2 state (a cold start: every stored
+        # alpha is the reference's initial value, Initialize.h:81-88)
+        cold = {}
+        for mode in (AlphaMode.WARM_RESIDUAL, AlphaMode.LEARNED, AlphaMode.NEWTON_RESIDUAL):
+            a.set_populations(f0)
+            a.set_alpha(np.full(a.N, 2.0))
+            a.reset_elbm_stats()
+            a.set_alpha_mode(mode, tol)
+            a.iterate(1)
+            cold[mode] = a.elbm_stats()
     fp, fneq = pulled_state(oracle, oc, f0)
     newton_cells, ref_fallbacks, worst, by_ref = 0, 0, 0.0, 0
     for k in range(fp.shape[1]):
@@ -161,6 +180,17 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
             else:  # the reference Newton decided this cell
                 assert abs(al - aref) <= 1e-9 * abs(aref), (k, mode, al, aref, g)
                 by_ref += 1
+        # plain Newton at the residual tolerance: a root with |G| <= tol, or
+        # the reference's alpha = 2 fallback
+        al = res[AlphaMode.NEWTON_RESIDUAL][0][k]
+        if al != 2.0:
+            g = abs(oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], al))
+            assert g <= 1.001 * tol, (k, al, g)
+            if b == oracle.ALPHA_NEWTON:
+                # the residual tolerance reaches the root to tol / |G'|; the
+                # reference Newton's iterate is within its last step (1e-8)
+                dg = abs(oracle.entropic_dg(int(lat), fp[:, k], fneq[:, k], aref))
+                assert abs(al - aref) <= 2.0 * tol / dg + 2e-8, (k, al, aref, dg)
     say(f"{newton_cells} solved cells ({ref_fallbacks} oracle fallbacks); worst accepted "
         f"|G(alpha)| {worst:.3e} (tol {tol:.0e}); {by_ref} cell-solves left to the reference")
     assert newton_cells > 0.5 * fp.shape[1]
@@ -172,9 +202,155 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
     # all three solvers find the same root to within the residual tolerance
     fN, fL = res[AlphaMode.NEWTON][2], res[AlphaMode.LEARNED][2]
     assert np.max(np.abs(fL - fN)) <= 1e-6 * np.max(np.abs(fN))
+    fR = res[AlphaMode.NEWTON_RESIDUAL][2]
+    assert np.max(np.abs(fR - fN)) <= 1e-6 * np.max(np.abs(fN))
     # what the network changes: evaluations per solved cell vs the same solver
-    # started from the stored alpha
+    # started from the stored alpha (reported), and from the alpha = 2 state,
+    # where the stored alpha is a poor start and the network must help
     ev_l = st_l["newton_iterations"] / max(1, st_l["newton"])
     ev_w = st_w["newton_iterations"] / max(1, st_w["newton"])
-    say(f"network gain: {ev_w:.3f} -> {ev_l:.3f} evaluations per solved cell")
-    assert ev_l <= ev_w + 1e-9
+    say(f"network gain (warm state): {ev_w:.3f} -> {ev_l:.3f} evaluations per solved cell")
+    per = {m: s["newton_iterations"] / max(1, s["newton"]) for m, s in cold.items()}
+    acc = {m: s["learned_accepted"] / max(1, s["newton"]) for m, s in cold.items()}
+    say(f"cold start (alpha = 2): evaluations per solved cell {per}; accepted by the "
+        f"correction {acc}")
+    # Plain Newton from alpha = 2 needs a second evaluation wherever |G(2)| >
+    # tol; the network's start is accepted or certified at the first.
+    assert per[AlphaMode.LEARNED] < 0.8 * per[AlphaMode.NEWTON_RESIDUAL]
+    if lat == LatticeType.D3Q27:
+        # In the D3Q27 noise case the previous step's alpha is a poor start
+        # for about one cell in seven: one certified correction from it fails
+        # there (the cell then pays the reference Newton as well), from the
+        # network's start it succeeds.  (In the D2Q9 case every alpha stays
+        # within ~1e-3 of the root and the correction from any of these starts
+        # is certified.)
+        assert ev_l < 0.8 * ev_w
+        acc_l = st_l["learned_accepted"] / max(1, st_l["newton"])
+        acc_w = st_w["learned_accepted"] / max(1, st_w["newton"])
+        assert acc_l > acc_w + 0.1
+        assert per[AlphaMode.LEARNED] < per[AlphaMode.WARM_RESIDUAL]
+
+
+ALL_MODES = (AlphaMode.NEWTON, AlphaMode.NEWTON_RESIDUAL, AlphaMode.WARM_RESIDUAL,
+             AlphaMode.LEARNED)
+
+
+def test_c4_fp32_build_every_alpha_mode(oracle):
+    """C4 -- D2Q9 ELBM, Guo + Kolmogorov k = 4 (A = 1e-5), tau = 0.50003,
+    noise 1e-3 -- in the build its throughput is timed in: shifted FP32
+    storage, FP32 compute, FP64 alpha solve (the learned mode with its f16
+    matrix-core inference fused in the step), at 128^2 so the FP64 oracle
+    runs alongside (lambda = 128/4).
+
+    1. 100 reference-Newton steps from the oracle's initial field with the
+       sample log on, then the network is trained on the logged solves.
+    2. Every solver runs the same 100 steps from the same start; f, u_h,
+       alpha, E and Omega are held to the FP64 oracle's 100 steps with the
+       bounds below (FP32 compute: the alpha solve sees fneq rounded in
+       FP32, so alpha differs from the FP64 root where fneq is small).
+    3. One step from the NEWTON state of step 100 in every mode, per cell:
+       cells outside the oracle's Newton branch carry NEWTON's alpha
+       bitwise (the prelude is shared); solved cells carry NEWTON's alpha
+       bitwise (the deferred reference Newton) or a root the residual
+       tolerance reaches from it, |alpha - alpha_N| <= 2 tol / |G'(alpha_N)| +
+       2e-8 (alpha_N is within the reference Newton's last step, <= 1e-8, of
+       the root; G' evaluated by the oracle)."""
+    n, tol = 128, 1e-10
+    base = dict(lattice=LatticeType.D2Q9, collision=CollisionType.ELBM,
+                forcing_scheme=ForcingSchemeType.Guo, force=ForceType.Kolmogorov, nx=n, ny=n,
+                nz=1, tau=0.50003, amplitude=(1e-5, 0, 0), wavelength=(n / 4, 1, 1),
+                residual_tol=tol)
+    cfg = Config(storage=Precision.F32, compute=Precision.F32, shifted=True, **base)
+    oc = oracle.Config(lattice=3, collision=2, forcing_scheme=2, force=6, equilibrium=3,
+                       nx=n, ny=n, nz=1, tau=0.50003, amplitude=(1e-5, 0, 0),
+                       wavelength=(n / 4, 1, 1))
+    f0 = oracle.init_equilibrium(oc, np.ones(oc.cells), oracle.noise_velocity(oc, 42, 1e-3))
+    ref = oracle.run(oc, f0, 100)
+    m_ref, e_ref, w_ref = oracle.reduce(oc, ref.rho, ref.u)
+    say(f"case c4 fp32 build 128^2; oracle branches {ref.branch.tolist()}")
+    runs = {}
+    with Algorithm(cfg) as a:
+        a.set_populations(f0)
+        a.sample_log_enable(400000, 1.0, 5)
+        a.iterate(100, store=True)
+        a.sample_log_disable()
+        runs[AlphaMode.NEWTON] = (a.populations(), a.store_fields(), a.reduce(), a.elbm_stats())
+        f100, al100 = a.populations(), a.alpha()
+        x, al = a.sample_log()
+        say(f"logged {len(al)} FP32-build solves")
+        l0, l1 = a.mlp_train(1500, 1024, 2e-3, seed=1)
+        say(f"training loss {l0:.4e} -> {l1:.4e}")
+        assert l1 < 0.5 * l0
+        for mode in ALL_MODES[1:]:
+            a.set_populations(f0)
+            a.set_alpha(np.full(a.N, 2.0))
+            a.reset_elbm_stats()
+            a.set_alpha_mode(mode, tol)
+            a.iterate(100, store=True)
+            runs[mode] = (a.populations(), a.store_fields(), a.reduce(), a.elbm_stats())
+        one = {}
+        for mode in ALL_MODES:
+            a.set_populations(f100)
+            a.set_alpha(al100)
+            a.reset_elbm_stats()
+            a.set_alpha_mode(mode, tol)
+            a.iterate(1, store=True)
+            one[mode] = (a.store_fields()[2], a.elbm_stats())
+    umax = np.max(np.abs(ref.u))
+    solved = ref.alpha != 2.0
+    for mode, (f, (rho, uh, alf, F), (m, e, w), st) in runs.items():
+        ef = rel_err(f, ref.f)
+        eu = float(np.max(np.abs(uh - ref.u))) / umax
+        ea = float(np.max(np.abs(alf - ref.alpha)))
+        ea_med = float(np.median(np.abs(alf - ref.alpha)[solved]))
+        say(f"{mode.name}: f {ef:.2e} u_h {eu:.2e} alpha max {ea:.2e} median(solved) "
+            f"{ea_med:.2e}; E {abs(e - e_ref) / e_ref:.2e} Omega {abs(w - w_ref) / w_ref:.2e} "
+            f"mass {abs(m - m_ref) / m_ref:.2e}; {st}")
+        assert ef <= 1e-5 and eu <= 1e-3
+        assert abs(m - m_ref) <= 1e-6 * m_ref
+        # The reference Newton converges to |step| <= 1e-8, so its energy and
+        # enstrophy track the oracle to FP32-compute level.  The residual
+        # modes accept any alpha with |G| <= 1e-10, up to tol / |G'| from the
+        # root (1e-4 of alpha where fneq is small); the relaxation then
+        # differs by that much per step, which moves E and Omega by ~1e-5
+        # (plain Newton and stored-alpha starts) to ~1e-4 (network starts,
+        # which land farther inside the tolerance) over 100 steps.
+        eb = 1e-5 if mode == AlphaMode.NEWTON else 5e-4
+        assert abs(e - e_ref) <= eb * e_ref and abs(w - w_ref) <= eb * w_ref
+        assert ea_med <= 1e-3
+        np.testing.assert_array_equal(F[0], ref.force[0].astype(np.float32).astype(np.float64))
+    fp, fneq = pulled_state(oracle, oc, f100)
+    aN = one[AlphaMode.NEWTON][0]
+    worst = {m: 0.0 for m in ALL_MODES[1:]}
+    same = {m: 0 for m in ALL_MODES[1:]}
+    nsolved, prelude_cells, borderline = 0, 0, 0
+    for k in range(fp.shape[1]):
+        _, b, _ = oracle.elbm_alpha(3, fp[:, k], fneq[:, k], al100[k])
+        if b in (oracle.ALPHA_SMALL, oracle.ALPHA_CAPPED):
+            # decided by the shared prelude: every mode gives the same alpha,
+            # unless FP32 rounding of fneq put the cell on the other side of a
+            # branch test than the FP64 oracle (counted, bounded below)
+            prelude_cells += 1
+            borderline += any(one[m][0][k] != aN[k] for m in ALL_MODES[1:])
+            continue
+        if b != oracle.ALPHA_NEWTON or aN[k] == 2.0:
+            continue  # a fallback of the oracle or of the GPU's reference Newton
+        nsolved += 1
+        for mode in ALL_MODES[1:]:
+            am = one[mode][0][k]
+            if am == aN[k]:
+                same[mode] += 1
+                continue
+            if mode == AlphaMode.NEWTON_RESIDUAL and am == 2.0:
+                continue  # its own alpha = 2 fallback (no root in 50 evaluations)
+            dg = abs(oracle.entropic_dg(3, fp[:, k], fneq[:, k], aN[k]))
+            reach = 2.0 * tol / dg + 2e-8
+            worst[mode] = max(worst[mode], abs(am - aN[k]) / reach)
+            assert abs(am - aN[k]) <= reach, (k, mode.name, am, aN[k], reach)
+    say(f"{nsolved} solved cells; NEWTON's alpha bitwise {same}; worst |dalpha| / reach "
+        f"{worst}; {prelude_cells} prelude cells, {borderline} borderline")
+    assert nsolved > 0.3 * fp.shape[1]
+    assert borderline <= max(3, 1e-3 * fp.shape[1])
+    for mode, (_, st) in one.items():
+        per = st["newton_iterations"] / max(1, st["newton"])
+        say(f"one step {mode.name}: {st}; evaluations per solved cell {per:.3f}")

@@ -144,12 +144,14 @@ def test_fp32_compute_tracks_fp64():
     assert rel(out[1][1], out[0][1]) <= 1e-4   # velocity
 
 
-@pytest.mark.parametrize("name,c", [CASES[0], CASES[3]], ids=[IDS[0], IDS[3]])
+@pytest.mark.parametrize("name,c", [CASES[0], CASES[1], CASES[3]],
+                         ids=[IDS[0], IDS[1], IDS[3]])
 def test_reinitialised_handle_matches_a_fresh_one(name, c):
     """The force field is rebuilt only when the fields it reads change (a
-    storing step, an initialiser, a restart).  A handle re-initialised in the
-    middle of a run must give the same steps, bit for bit, as a fresh handle
-    given the same initial field."""
+    storing step, an initialiser, a restart), and an initialiser restarts the
+    entropic alpha field at 2 (initAlpha, Initialize.h:81-88).  A handle
+    re-initialised in the middle of a run must give the same steps, bit for
+    bit, as a fresh handle given the same initial field."""
     print(f"case spectral force re-init {name}", flush=True)
     with Algorithm(c) as a:
         a.init_noise(1.0, 2e-2, 3)
@@ -166,6 +168,21 @@ def test_reinitialised_handle_matches_a_fresh_one(name, c):
         F_fresh = b.store_fields()[3]
     np.testing.assert_array_equal(F_re, F_fresh)
     np.testing.assert_array_equal(f_re, f_fresh)
+
+
+def test_restart_from_file_resets_alpha(tmp_path):
+    """A restart from a distribution file starts the entropic solve from
+    alpha = 2, as the reference's initAlpha does on every start
+    (Initialize.h:81-88, 119-145)."""
+    print("case elbm restart alpha", flush=True)
+    c = CASES[1][1]
+    with Algorithm(c) as a:
+        a.init_noise(1.0, 2e-2, 3)
+        a.iterate(5)
+        assert np.any(a.alpha() != 2.0)
+        a.write_distribution_hdf5(str(tmp_path / "d.h5"))
+        a.read_distribution_hdf5(str(tmp_path / "d.h5"))
+        np.testing.assert_array_equal(a.alpha(), np.full(c.cells, 2.0))
 
 
 def test_restart_fields_feed_energy_removal():

@@ -637,6 +637,51 @@ def hdf5_bench(n: int) -> dict:
     return out
 
 
+def stream_ceiling(K: int, W: int) -> dict:
+    """The headline step kernel against the bandwidth probes of the library
+    (lbm_bandwidth_probe) on the same D3Q19 256^3 FP32 lattices: the step's
+    access pattern without its arithmetic (2 Q streams), its Q reads alone,
+    its Q writes alone, and one contiguous copy of the same bytes.  Average
+    launch time from HIP events on the stream the kernels run on."""
+    import metalbm_amd as m
+    wl = WORKLOADS[DEFAULT]
+    cfg = make_config(wl["cfg"])
+    hip = Hip()
+    a = m.Algorithm(cfg)
+    s = hip.stream()
+    a.set_stream(s.value)
+    init(a, wl["init"])
+    N, Q = cfg.cells, cfg.q
+    pop = N * SIZES[wl["cfg"]["storage"]]
+
+    def timed(run):
+        run(W)
+        a.synchronize()
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, s)
+        run(K)
+        hip.record(e1, s)
+        return hip.elapsed_ms(e0, e1) / K
+
+    out = {"metric": "achieved HBM bandwidth of the step kernel vs probes of its access pattern",
+           "unit": "GB/s", "steps": K, "warmup": W,
+           "config": {"workload": DEFAULT, "cells": N, "bytes_per_population": pop}}
+    ms = timed(lambda k: a.iterate(k))
+    out["k_bgk_vec"] = {"us": round(ms * 1e3, 2), "bytes": 2 * Q * pop,
+                        "GB/s": round(2 * Q * pop / ms / 1e6, 1)}
+    for kind, name, nbytes in ((0, "pull_copy", 2 * Q * pop), (1, "read", Q * pop),
+                               (2, "write", Q * pop), (3, "copy", 2 * Q * pop)):
+        ms = timed(lambda k, kind=kind: a.bandwidth_probe(kind, k))
+        out[name] = {"us": round(ms * 1e3, 2), "bytes": nbytes,
+                     "GB/s": round(nbytes / ms / 1e6, 1)}
+        log(f"probe {name}: {out[name]}")
+    a.close()
+    out["kernel_vs_pull_copy"] = round(out["k_bgk_vec"]["GB/s"] / out["pull_copy"]["GB/s"], 4)
+    out["kernel_vs_copy"] = round(out["k_bgk_vec"]["GB/s"] / out["copy"]["GB/s"], 4)
+    out["peak"] = PEAK_HBM_GBS
+    return out
+
+
 def traffic_of(name):
     """HBM bytes per launch of the workload's step kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_<name>.txt,
@@ -678,7 +723,13 @@ def main(argv=None) -> int:
                     help="time the HDF5 field / distribution files of D3Q19 n^3 "
                          "(prints its own JSON line)")
     ap.add_argument("--hdf5-n", type=int, default=128)
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="the headline step kernel against bandwidth probes of its own "
+                         "access pattern (prints its own JSON line)")
     args = ap.parse_args(argv)
+    if args.stream_ceiling:
+        print(json.dumps(stream_ceiling(args.steps or 100, args.warmup)), flush=True)
+        return 0
     if args.hdf5:
         print(json.dumps(hdf5_bench(args.hdf5_n)), flush=True)
         return 0

@@ -234,6 +234,25 @@ int lbm_get_fields(lbm_handle *h, double *rho, double *u_hydro,
  * differences and periodic wrap.  Any output pointer may be NULL. */
 int lbm_reduce(lbm_handle *h, double *mass, double *energy, double *enstrophy);
 
+/* ---- Bandwidth probes ------------------------------------------------------
+ * Measurement aid, no reference counterpart: enqueues nsteps launches of a
+ * probe that has the BGK vector kernel's memory access pattern -- 16 bytes
+ * per lane, its tile order, per-row buffer resources, non-temporal stores --
+ * and none of its arithmetic, over the handle's two lattices (ping-pong), so
+ * that the kernel's achieved bandwidth can be read against what the same
+ * pattern reaches on this GPU.  Kinds:
+ *   LBM_PROBE_PULL_COPY  the Q reads at the pull source rows and the Q writes
+ *                        of the step (2 Q streams);
+ *   LBM_PROBE_READ       the Q reads only;
+ *   LBM_PROBE_WRITE      the Q writes only;
+ *   LBM_PROBE_COPY       one contiguous copy of the same 2 Q N sizeof(S)
+ *                        bytes (one read and one write stream).
+ * Needs the contiguous extent (nz in 3D, ny in 2D) to hold a multiple of 16
+ * bytes.  Overwrites the populations: initialise again before stepping. */
+enum { LBM_PROBE_PULL_COPY = 0, LBM_PROBE_READ = 1, LBM_PROBE_WRITE = 2,
+       LBM_PROBE_COPY = 3 };
+int lbm_bandwidth_probe(lbm_handle *h, int kind, int nsteps);
+
 /* Entropic statistics accumulated since creation or the last reset. */
 int lbm_get_elbm_stats(lbm_handle *h, lbm_elbm_stats *out);
 int lbm_reset_elbm_stats(lbm_handle *h);

@@ -270,6 +270,11 @@ class Algorithm:
                                      ctypes.byref(w)))
         return m.value, e.value, w.value
 
+    def bandwidth_probe(self, kind: int, nsteps: int = 1) -> None:
+        """nsteps launches of bandwidth probe `kind` (LBM_PROBE_*: 0 pull copy,
+        1 read, 2 write, 3 contiguous copy); overwrites the populations."""
+        self._check(lib().lbm_bandwidth_probe(self._h, int(kind), int(nsteps)))
+
     def elbm_stats(self) -> dict:
         s = LbmElbmStatsC()
         self._check(lib().lbm_get_elbm_stats(self._h, ctypes.byref(s)))

@@ -39,6 +39,7 @@ SOURCES = {
     "capi.hip": ["-ffp-contract=off"],
     "hdf5_io.cpp": ["-ffp-contract=off"],
     "spectral.hip": ["-ffp-contract=off"],
+    "probe.hip": ["-ffp-contract=off"],
 }
 
 

@@ -17,6 +17,7 @@
 #include "aux_kernels.hpp"
 #include "capi_internal.hpp"
 #include "mlp_train.hpp"
+#include "probe.hpp"
 #include "registry.hpp"
 
 using namespace lbm;
@@ -1059,6 +1060,21 @@ int lbm_reduce(lbm_handle* h, double* mass, double* energy, double* enstrophy) {
   return LBM_OK;
 }
 
+int lbm_bandwidth_probe(lbm_handle* h, int kind, int nsteps) {
+  if (!h) return LBM_E_INVALID;
+  if (kind < LBM_PROBE_PULL_COPY || kind > LBM_PROBE_COPY)
+    return fail(h, LBM_E_INVALID, "probe kind must be PULL_COPY (0), READ (1), WRITE (2) or COPY (3)");
+  if (nsteps < 0) return fail(h, LBM_E_INVALID, "nsteps must be >= 0");
+  if ((size_t)h->nk * h->elem % 16 != 0)
+    return fail(h, LBM_E_INVALID, "the probes need a contiguous extent of a multiple of 16 bytes");
+  for (int s = 0; s < nsteps; ++s) {
+    HIP_TRY(h, launch_bandwidth_probe(kind, h->cfg.lattice, h->f[h->cur], h->f[1 - h->cur],
+                                      h->cfg.nx, h->cfg.ny, h->cfg.nz, (int)h->elem, h->stream));
+    h->cur = 1 - h->cur;
+  }
+  return LBM_OK;
+}
+
 int lbm_get_elbm_stats(lbm_handle* h, lbm_elbm_stats* out) {
   if (!h || !out) return LBM_E_INVALID;
   std::memset(out, 0, sizeof(*out));
@@ -1251,10 +1267,12 @@ namespace {
 
 // Device buffers for one gradient / training pass over n samples.
 struct TrainBufs {
-  float *X = nullptr, *T = nullptr, *G = nullptr, *g = nullptr, *loss = nullptr;
-  int* idx = nullptr;
+  float *X = nullptr, *T = nullptr, *partial = nullptr, *g = nullptr, *loss = nullptr;
+  float* loss_partial = nullptr;
+  int* perm = nullptr;
   ~TrainBufs() {
-    for (void* p : {(void*)X, (void*)T, (void*)G, (void*)g, (void*)loss, (void*)idx})
+    for (void* p : {(void*)X, (void*)T, (void*)partial, (void*)g, (void*)loss,
+                    (void*)loss_partial, (void*)perm})
       if (p) (void)hipFree(p);
   }
 };
@@ -1279,24 +1297,26 @@ int lbm_mlp_gradient(lbm_handle* h, const float* x, const double* alpha, int n,
   if (int rc = targets_of(h, alpha, (size_t)n, t)) return rc;
   TrainBufs b;
   const int np = h->mlp_np;
+  const int nc = mlp_grad_chunks(n);
   HIP_TRY(h, hipMalloc(&b.X, sizeof(float) * (size_t)n * h->mlp_nin));
   HIP_TRY(h, hipMalloc(&b.T, sizeof(float) * (size_t)n));
-  HIP_TRY(h, hipMalloc(&b.G, sizeof(float) * (size_t)n * np));
+  HIP_TRY(h, hipMalloc(&b.partial, sizeof(float) * (size_t)nc * np));
+  HIP_TRY(h, hipMalloc(&b.loss_partial, sizeof(float) * (size_t)nc));
   HIP_TRY(h, hipMalloc(&b.g, sizeof(float) * (size_t)np));
-  HIP_TRY(h, hipMalloc(&b.loss, sizeof(float) * (size_t)n));
+  HIP_TRY(h, hipMalloc(&b.loss, sizeof(float)));
   HIP_TRY(h, hipMemcpy(b.X, x, sizeof(float) * (size_t)n * h->mlp_nin, hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(b.T, t.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
-  HIP_TRY(h, launch_mlp_sample_grad(h->cfg.lattice, h->mlp, b.X, b.T, nullptr, n, b.G,
-                                    b.loss, h->stream));
-  HIP_TRY(h, launch_mlp_sum(b.G, n, np, b.g, h->stream));
+  HIP_TRY(h, launch_mlp_grad(h->cfg.lattice, h->mlp, b.X, b.T, nullptr, n, b.partial,
+                             b.loss_partial, h->stream));
+  HIP_TRY(h, launch_mlp_reduce_adam(b.partial, b.loss_partial, nc, np, b.g, b.loss, 0,
+                                    nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                                    h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   HIP_TRY(h, hipMemcpy(grad, b.g, sizeof(float) * np, hipMemcpyDeviceToHost));
   if (loss) {
-    std::vector<float> l(n);
-    HIP_TRY(h, hipMemcpy(l.data(), b.loss, sizeof(float) * n, hipMemcpyDeviceToHost));
-    double s = 0.0;
-    for (float v : l) s += v;
-    *loss = s / (2.0 * n);
+    float l = 0.0f;
+    HIP_TRY(h, hipMemcpy(&l, b.loss, sizeof(float), hipMemcpyDeviceToHost));
+    *loss = (double)l / (2.0 * n);
   }
   return LBM_OK;
 }
@@ -1448,52 +1468,54 @@ int lbm_mlp_train(lbm_handle* h, int iterations, int batch, double lr, uint64_t 
   std::vector<float> t;
   if (int rc = targets_of(h, al.data(), (size_t)n, t)) return rc;
   const int B = (int)std::min<int64_t>(batch, n);
+  const int nc = mlp_grad_chunks(B);
   TrainBufs b;
   HIP_TRY(h, hipMalloc(&b.T, sizeof(float) * (size_t)n));
-  HIP_TRY(h, hipMalloc(&b.G, sizeof(float) * (size_t)B * np));
+  HIP_TRY(h, hipMalloc(&b.partial, sizeof(float) * (size_t)nc * np));
+  HIP_TRY(h, hipMalloc(&b.loss_partial, sizeof(float) * (size_t)nc));
   HIP_TRY(h, hipMalloc(&b.g, sizeof(float) * (size_t)np));
-  HIP_TRY(h, hipMalloc(&b.loss, sizeof(float) * (size_t)B));
-  HIP_TRY(h, hipMalloc(&b.idx, sizeof(int) * (size_t)B));
+  HIP_TRY(h, hipMalloc(&b.loss, sizeof(float)));
+  HIP_TRY(h, hipMalloc(&b.perm, sizeof(int) * (size_t)n));
   HIP_TRY(h, hipMemcpy(b.T, t.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice));
-  // Seeded Fisher-Yates permutation, consumed B at a time.
+  // Seeded Fisher-Yates permutation, consumed B at a time; each new
+  // permutation is copied to the device once and the minibatches index it.
   std::vector<int> perm((size_t)n);
   for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
   uint64_t draw = 0;
-  auto shuffle = [&]() {
+  auto shuffle = [&]() -> int {
     for (int64_t i = n - 1; i > 0; --i) {
       const int64_t j = (int64_t)(mix64(seed, draw++) % (uint64_t)(i + 1));
       std::swap(perm[i], perm[j]);
     }
+    // the previous permutation may still be read by queued kernels
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    HIP_TRY(h, hipMemcpy(b.perm, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice));
+    return LBM_OK;
   };
-  shuffle();
+  if (int rc = shuffle()) return rc;
   int64_t pos = 0;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  std::vector<float> lossv(B);
   for (int it = 0; it < iterations; ++it) {
     if (pos + B > n) {
-      shuffle();
+      if (int rc = shuffle()) return rc;
       pos = 0;
     }
-    HIP_TRY(h, hipMemcpyAsync(b.idx, perm.data() + pos, sizeof(int) * B,
-                              hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, launch_mlp_grad(h->cfg.lattice, h->mlp, h->log_x, b.T, b.perm + pos, B,
+                               b.partial, b.loss_partial, h->stream));
     pos += B;
-    HIP_TRY(h, launch_mlp_sample_grad(h->cfg.lattice, h->mlp, h->log_x, b.T, b.idx, B,
-                                      b.G, b.loss, h->stream));
-    HIP_TRY(h, launch_mlp_sum(b.G, B, np, b.g, h->stream));
     h->adam_t += 1;
     const float c1 = (float)(1.0 - std::pow(0.9, h->adam_t));
     const float c2 = (float)(1.0 - std::pow(0.999, h->adam_t));
     h->mlp_dirty = true;
-    HIP_TRY(h, launch_mlp_adam(h->mlp, b.g, h->mlp_m, h->mlp_v, np, (float)lr, b1, b2,
-                               eps, c1, c2, h->stream));
+    HIP_TRY(h, launch_mlp_reduce_adam(b.partial, b.loss_partial, nc, np, b.g, b.loss, 1,
+                                      h->mlp, h->mlp_m, h->mlp_v, (float)lr, b1, b2, eps, c1,
+                                      c2, h->stream));
     if ((it == 0 && loss_first) || (it == iterations - 1 && loss_last)) {
-      HIP_TRY(h, hipMemcpyAsync(lossv.data(), b.loss, sizeof(float) * B,
-                                hipMemcpyDeviceToHost, h->stream));
+      float l = 0.0f;
+      HIP_TRY(h, hipMemcpyAsync(&l, b.loss, sizeof(float), hipMemcpyDeviceToHost, h->stream));
       HIP_TRY(h, hipStreamSynchronize(h->stream));
-      double s = 0.0;
-      for (float v : lossv) s += v;
-      if (it == 0 && loss_first) *loss_first = s / (2.0 * B);
-      if (it == iterations - 1 && loss_last) *loss_last = s / (2.0 * B);
+      if (it == 0 && loss_first) *loss_first = (double)l / (2.0 * B);
+      if (it == iterations - 1 && loss_last) *loss_last = (double)l / (2.0 * B);
     }
   }
   HIP_TRY(h, hipStreamSynchronize(h->stream));

@@ -234,6 +234,18 @@ int lbm_get_fields(lbm_handle *h, double *rho, double *u_hydro,
  * differences and periodic wrap.  Any output pointer may be NULL. */
 int lbm_reduce(lbm_handle *h, double *mass, double *energy, double *enstrophy);
 
+/* lbm_reduce without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * the same results, bit for bit, with the partial sums in a static device
+ * array of the library (1.5 MiB per GPU, held until the process ends) instead
+ * of the buffer lbm_reduce allocates on first use.  Grids of more than 2^29
+ * cells exceed that array and are rejected (LBM_E_INVALID).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (two host
+ * threads, or handles on different streams) race. */
+int lbm_reduce_noalloc(lbm_handle *h, double *mass, double *energy,
+                       double *enstrophy);
+
 /* ---- Bandwidth probes ------------------------------------------------------
  * Measurement aid, no reference counterpart: enqueues nsteps launches of a
  * probe that has the BGK vector kernel's memory access pattern -- 16 bytes

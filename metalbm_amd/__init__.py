@@ -275,6 +275,14 @@ class Algorithm:
         1 read, 2 write, 3 contiguous copy); overwrites the populations."""
         self._check(lib().lbm_bandwidth_probe(self._h, int(kind), int(nsteps)))
 
+    def reduce_noalloc(self):
+        """reduce() through lbm_reduce_noalloc: no device allocation (static
+        scratch of the library; grids up to 2^29 cells)."""
+        m, e, w = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lbm_reduce_noalloc(self._h, ctypes.byref(m), ctypes.byref(e),
+                                             ctypes.byref(w)))
+        return m.value, e.value, w.value
+
     def elbm_stats(self) -> dict:
         s = LbmElbmStatsC()
         self._check(lib().lbm_get_elbm_stats(self._h, ctypes.byref(s)))

@@ -28,7 +28,7 @@ EXPORTS = (
     "lbm_sample_log_count", "lbm_sample_log_copy", "lbm_mlp_train", "lbm_set_alpha_mode",
     "lbm_get_config", "lbm_write_fields_hdf5", "lbm_write_distribution_hdf5",
     "lbm_read_distribution_hdf5", "lbm_spectral_shells", "lbm_vorticity", "lbm_spectra",
-    "lbm_kernel_shape", "lbm_bandwidth_probe",
+    "lbm_kernel_shape", "lbm_reduce_noalloc", "lbm_bandwidth_probe",
 )
 
 
@@ -99,6 +99,7 @@ def lib() -> ctypes.CDLL:
     L.lbm_step.argtypes = [H, ctypes.c_int, ctypes.c_int]
     L.lbm_get_fields.argtypes = [H, dp, dp, dp, dp]
     L.lbm_reduce.argtypes = [H, dp, dp, dp]
+    L.lbm_reduce_noalloc.argtypes = [H, dp, dp, dp]
     L.lbm_bandwidth_probe.argtypes = [H, ctypes.c_int, ctypes.c_int]
     L.lbm_get_elbm_stats.argtypes = [H, ctypes.POINTER(LbmElbmStatsC)]
     L.lbm_reset_elbm_stats.argtypes = [H]

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,7 @@ const Registry& registry() {
 
 struct lbm_handle {
   lbm_config cfg{};
+  int device = 0;           // the HIP device current at lbm_create
   int Q = 0, D = 0, N = 0;
   int nk = 0, rows = 0, block = 256, tiles_per_row = 1;
   int vec = 0;              // cells per lane of the vector kernel (0: cell kernel)
@@ -594,6 +596,11 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
   if (validate(cfg, why) != LBM_OK) return fail(nullptr, LBM_E_INVALID, why);
   lbm_handle* h = new lbm_handle();
   h->cfg = *cfg;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    (void)hipGetLastError();
+    delete h;
+    return fail(nullptr, LBM_E_HIP, "hipGetDevice failed: no usable HIP device");
+  }
   h->alpha_mode = cfg->collision == LBM_ELBM ? cfg->alpha_mode : LBM_ALPHA_NEWTON;
   h->residual_tol = cfg->residual_tol > 0.0 ? cfg->residual_tol : 1e-10;
   if (lattice_tables(cfg->lattice, h->w, h->scale, &h->Q, &h->D) != 0) {
@@ -1051,6 +1058,74 @@ int lbm_reduce(lbm_handle* h, double* mass, double* energy, double* enstrophy) {
   double* out_dev = h->reduce_buf + 3 * (size_t)reduce_blocks(h->N);
   HIP_TRY(h, launch_reduce(h->fields, h->fields + h->N, h->cfg.nx, h->cfg.ny,
                            h->cfg.nz, h->D, h->reduce_buf, out_dev, h->stream));
+  double out[3];
+  HIP_TRY(h, hipMemcpyAsync(out, out_dev, sizeof(out), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (mass) *mass = out[0];
+  if (energy) *energy = out[1] / (double)h->N;
+  if (enstrophy) *enstrophy = out[2] / (double)h->N;
+  return LBM_OK;
+}
+
+}  // extern "C"
+
+// ---- reduction without allocation --------------------------------------------
+namespace {
+
+// Partial sums of lbm_reduce_noalloc: 3 per reduction block for up to
+// kNoAllocBlocks blocks of 8192 cells (grids of up to 2^29 cells), then the
+// 3 results -- 1.5 MiB.  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code `scratch` only names the symbol: its
+// device address comes from hipGetSymbolAddress, resolved once per device.
+constexpr int kNoAllocBlocks = 65536;
+static __device__ __align__(256) double scratch[3 * kNoAllocBlocks + 3];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of the handle's GPU's scratch array (resolved on the
+// first call for that device, with the device current).
+int scratch_of(lbm_handle* h, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  const int dev = h->device;
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(h, LBM_E_INVALID, "lbm_reduce_noalloc: device index out of range");
+  if (!g_scratch[dev]) {
+    int cur = -1;
+    HIP_TRY(h, hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(h, hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(h, e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(h, LBM_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm_reduce_noalloc(lbm_handle* h, double* mass, double* energy, double* enstrophy) {
+  if (!h) return LBM_E_INVALID;
+  const int nb = reduce_blocks(h->N);
+  if (nb > kNoAllocBlocks)
+    return fail(h, LBM_E_INVALID,
+                "lbm_reduce_noalloc: the grid has more than 2^29 cells, beyond its static "
+                "scratch of 65536 reduction blocks (use lbm_reduce)");
+  if (!h->fields_valid)
+    return fail(h, LBM_E_STATE, "no storing step has run (lbm_step store_last)");
+  double* base = nullptr;
+  if (int rc = scratch_of(h, &base)) return rc;
+  // The partial sums are all rewritten by the first kernel before the second
+  // reads them, so nothing left from an earlier call needs clearing.
+  double* out_dev = base + 3 * (size_t)nb;
+  HIP_TRY(h, launch_reduce(h->fields, h->fields + h->N, h->cfg.nx, h->cfg.ny,
+                           h->cfg.nz, h->D, base, out_dev, h->stream));
   double out[3];
   HIP_TRY(h, hipMemcpyAsync(out, out_dev, sizeof(out), hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));

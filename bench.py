@@ -86,6 +86,22 @@ WORKLOADS = {
         init=("noise", 1e-3, 42), steps=200,
         learn=dict(log_steps=1000, fraction=1e-4, capacity=500000, iters=3000, batch=4096,
                    lr=2e-3)),
+    "d2q9-elbm-2048-newton-residual": dict(
+        desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, plain Newton at the "
+             "residual tolerance: from the stored alpha until |G| <= 1e-10 (at most 50 "
+             "evaluations, alpha = 2 outside (1, alphaMax)), one launch",
+        cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
+                 forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
+                 wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True,
+                 alpha_mode="NEWTON_RESIDUAL", residual_tol=1e-10),
+        init=("noise", 1e-3, 42), steps=200),
+    "d3q27-elbm-256-newton-residual": dict(
+        desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, plain Newton at the "
+             "residual tolerance: from the stored alpha until |G| <= 1e-10, one launch",
+        cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
+                 forcing_scheme="None_", force="None_", storage="F32", compute="F32",
+                 shifted=True, alpha_mode="NEWTON_RESIDUAL", residual_tol=1e-10),
+        init=("spectral", 0.1 / 3 ** 0.5, 16, 4.0, 42), steps=50),
     "d3q27-elbm-256-newton": dict(
         desc="D3Q27 SRT Entropic 256^3 decaying turbulence, nu=2e-5 (tau=0.50006), no force, "
              "solenoidal random-phase u0 with E(k) ~ k^4 exp(-2 (k/4)^2) on shells 1..16 "

@@ -450,6 +450,10 @@ StepArgs make_args(const lbm_handle* h) {
   a.nk = h->nk;
   a.tiles_per_row = h->tiles_per_row;
   a.N = h->N;
+  // Non-temporal loads in the BGK cell kernel where one workgroup covers a
+  // whole row and the two lattices far exceed the caches (>= 1 GiB).
+  a.nt_loads = (h->tiles_per_row == 1 &&
+                2.0 * h->Q * (double)h->N * (double)h->elem >= 1073741824.0) ? 1 : 0;
   const double beta = 1.0 / (2.0 * c.tau);    // Collision.h:122
   a.beta = beta;
   a.omega_f = 1. - 2. * beta;                 // Collision.h:145
@@ -988,8 +992,8 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
     if (!h->defer_list) HIP_TRY(h, hipMalloc(&h->defer_list, (size_t)h->N * sizeof(int)));
   }
   StepArgs a = make_args(h);
-  // Vector kernel: one wavefront per 64 V-cell segment, 4 per workgroup.
-  const int grid = h->vec ? (h->rows * h->tiles_per_row + 3) / 4
+  // Vector kernel: one wavefront per 64 V-cell segment, kVecWaves per workgroup.
+  const int grid = h->vec ? (h->rows * h->tiles_per_row + kVecWaves - 1) / kVecWaves
                           : h->rows * h->tiles_per_row;
   const bool field_force = h->ffield != nullptr;
   const bool time_dependent = h->cfg.force == LBM_FORCE_ENERGY_REMOVAL ||

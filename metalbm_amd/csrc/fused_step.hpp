@@ -37,7 +37,8 @@ __device__ __forceinline__ void store_fields(const StepArgs& a, int cell, int FS
 
 // FSX: the forcing scheme as configured (None vs Shan-Chen matters only for
 // the stored hydrodynamic velocity), passed at run time.
-template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK>
+// NT: non-temporal population loads (chosen per configuration by the host).
+template <class L, class S, class C, int FS, int EQ, bool SH, int BLOCK, bool NT>
 __global__ void __launch_bounds__(BLOCK) k_bgk(const StepArgs a, int FSX) {
   constexpr int Q = L::dimQ;
   constexpr bool REF = std::is_same_v<C, double> && !SH;
@@ -47,7 +48,7 @@ __global__ void __launch_bounds__(BLOCK) k_bgk(const StepArgs a, int FSX) {
   S* __restrict__ dst = static_cast<S*>(a.dst);
 
   C f[Q];
-  pull<L, S, C>(a, c, f);
+  pull<L, S, C, NT>(a, c, f);
   C F[3];
   force_of_cell<L, C>(a, c, F);
   const C of = C(a.omega_f), oe = C(a.omega_eq), gc = C(a.guo_coef);

@@ -78,6 +78,12 @@ struct WavePos {
 // Accesses beyond num_records return 0 / are dropped.
 constexpr int kBufferDword3 = 0x00020000;  // raw buffer, 32-bit data format
 
+// Cache policy of the population loads: non-temporal (each value is read
+// once per step from HBM; the shifted-row lines neighbouring wavefronts
+// share still meet in the L2).  Measured on the same probe: +3.8% over the
+// default policy, +8.5% together with single-wavefront groups.
+constexpr int kLoadPolicy = 2;
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base,
                                                           unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
@@ -90,7 +96,7 @@ template <>
 struct BufIO<16> {
   typedef unsigned raw __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
-    return __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, 0);
+    return __builtin_amdgcn_raw_buffer_load_b128(r, vo, so, kLoadPolicy);
   }
   static __device__ __forceinline__ void store(raw v, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
     __builtin_amdgcn_raw_buffer_store_b128(v, r, vo, so, 2 /* nt */);
@@ -100,7 +106,7 @@ template <>
 struct BufIO<8> {
   typedef unsigned raw __attribute__((ext_vector_type(2)));
   static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
-    return __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0);
+    return __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, kLoadPolicy);
   }
   static __device__ __forceinline__ void store(raw v, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
     __builtin_amdgcn_raw_buffer_store_b64(v, r, vo, so, 2);
@@ -110,7 +116,7 @@ template <>
 struct BufIO<4> {
   typedef unsigned raw;
   static __device__ __forceinline__ raw load(__amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, kLoadPolicy);
   }
   static __device__ __forceinline__ void store(raw v, __amdgpu_buffer_rsrc_t r, unsigned vo, int so) {
     __builtin_amdgcn_raw_buffer_store_b32(v, r, vo, so, 2);
@@ -200,12 +206,12 @@ __device__ __forceinline__ void store_vec(const StepArgs& a, const WavePos& w,
 }
 
 template <class L, class S, class C, int FS, int EQ, bool SH, int V>
-__global__ void __launch_bounds__(256) k_bgk_vec(const StepArgs a, int FSX) {
+__global__ void __launch_bounds__(64 * kVecWaves) k_bgk_vec(const StepArgs a, int FSX) {
   constexpr int Q = L::dimQ, D = L::dimD;
   constexpr bool REF = std::is_same_v<C, double> && !SH;
   // threadIdx.x >> 6 is the wavefront index: uniform, which readfirstlane
   // makes visible to the compiler so all row addressing stays in SGPRs.
-  const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 +
+  const int wave = xcd_tile(blockIdx.x, gridDim.x) * kVecWaves +
                    __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int segs = a.tiles_per_row;   // segments of 64 V cells per row
   const int row = wave / segs;

@@ -13,9 +13,14 @@ namespace lbm {
 template <class L, class S, class C, int FS, int EQ, bool SH, int CO>
 hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
                        hipStream_t st) {
-  if constexpr (CO == (int)CollisionType::BGK)
-    hipLaunchKernelGGL((k_bgk<L, S, C, FS, EQ, SH, 256>), dim3(grid),
-                       dim3(block), 0, st, a, fsx);
+  if constexpr (CO == (int)CollisionType::BGK) {
+    if (a.nt_loads)
+      hipLaunchKernelGGL((k_bgk<L, S, C, FS, EQ, SH, 256, true>), dim3(grid),
+                         dim3(block), 0, st, a, fsx);
+    else
+      hipLaunchKernelGGL((k_bgk<L, S, C, FS, EQ, SH, 256, false>), dim3(grid),
+                         dim3(block), 0, st, a, fsx);
+  }
   else if (a.alpha_mode == 0)
     hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 0>), dim3(grid),
                        dim3(block), 0, st, a, fsx);
@@ -48,7 +53,7 @@ template <class L, class S, class C, int FS, int EQ, bool SH, int V>
 hipError_t launch_step_vec(const StepArgs& a, int fsx, int block, int grid,
                            hipStream_t st) {
   (void)block;
-  hipLaunchKernelGGL((k_bgk_vec<L, S, C, FS, EQ, SH, V>), dim3(grid), dim3(256),
+  hipLaunchKernelGGL((k_bgk_vec<L, S, C, FS, EQ, SH, V>), dim3(grid), dim3(64 * kVecWaves),
                      0, st, a, fsx);
   return hipGetLastError();
 }

@@ -2,10 +2,10 @@
 // and none of its arithmetic: the measured ceiling the roofline of the
 // vector step kernel (k_bgk_vec) is read against.  No reference counterpart.
 //
-// Every probe uses k_bgk_vec's work decomposition: 16 bytes per lane, one
-// 64-lane wavefront per row segment of 64 lanes, four wavefronts per
-// workgroup, the XCD-aware tile order, per-row buffer resources, and
-// non-temporal stores.  Kinds (LBM_PROBE_*):
+// Every probe uses k_bgk_vec's work decomposition and cache policies: 16
+// bytes per lane, one 64-lane wavefront per row segment of 64 lanes,
+// kVecWaves wavefronts per workgroup, the XCD-aware tile order, per-row
+// buffer resources, non-temporal loads and stores.  Kinds (LBM_PROBE_*):
 //   PULL_COPY  each lane reads its Q rows at the pull sources (x - c_x,
 //              y - c_y; the +-1 shift along the contiguous axis is left out:
 //              it costs no HBM traffic, its lines are shared with the
@@ -44,10 +44,12 @@ struct ProbeArgs {
   long long pop_bytes;  // bytes of one population array (N * elem)
 };
 
+constexpr int kLoadNT = 2;  // the step kernel's load policy (non-temporal)
+
 template <class L, int KIND>
-__global__ void __launch_bounds__(256) k_probe(const ProbeArgs a) {
+__global__ void __launch_bounds__(64 * kVecWaves) k_probe(const ProbeArgs a) {
   constexpr int Q = L::dimQ, D = L::dimD;
-  const int wave = xcd_tile(blockIdx.x, gridDim.x) * 4 +
+  const int wave = xcd_tile(blockIdx.x, gridDim.x) * kVecWaves +
                    __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if constexpr (KIND == LBM_PROBE_COPY) {
@@ -60,7 +62,8 @@ __global__ void __launch_bounds__(256) k_probe(const ProbeArgs a) {
     u32x4 v[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i)
-      v[i] = __builtin_amdgcn_raw_buffer_load_b128(rs, (unsigned)(i * 1024 + lane * 16), 0, 0);
+      v[i] = __builtin_amdgcn_raw_buffer_load_b128(rs, (unsigned)(i * 1024 + lane * 16), 0,
+                                                   kLoadNT);
 #pragma unroll
     for (int i = 0; i < Q; ++i)
       __builtin_amdgcn_raw_buffer_store_b128(v[i], rd, (unsigned)(i * 1024 + lane * 16), 0, 2);
@@ -99,7 +102,7 @@ __global__ void __launch_bounds__(256) k_probe(const ProbeArgs a) {
           src_row = xs;
         }
         const char* p = a.src + (long long)i * a.pop_bytes + (long long)src_row * row_bytes;
-        v[i] = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(p, row_bytes), voff, 0, 0);
+        v[i] = __builtin_amdgcn_raw_buffer_load_b128(row_rsrc(p, row_bytes), voff, 0, kLoadNT);
       });
     } else {
 #pragma unroll
@@ -129,16 +132,20 @@ template <class L>
 hipError_t launch_for(int kind, const ProbeArgs& a, int grid, hipStream_t st) {
   switch (kind) {
     case LBM_PROBE_PULL_COPY:
-      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_PULL_COPY>), dim3(grid), dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_PULL_COPY>), dim3(grid), dim3(64 * kVecWaves), 0, st,
+                         a);
       break;
     case LBM_PROBE_READ:
-      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_READ>), dim3(grid), dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_READ>), dim3(grid), dim3(64 * kVecWaves), 0, st,
+                         a);
       break;
     case LBM_PROBE_WRITE:
-      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_WRITE>), dim3(grid), dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_WRITE>), dim3(grid), dim3(64 * kVecWaves), 0, st,
+                         a);
       break;
     default:
-      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_COPY>), dim3(grid), dim3(256), 0, st, a);
+      hipLaunchKernelGGL((k_probe<L, LBM_PROBE_COPY>), dim3(grid), dim3(64 * kVecWaves), 0, st,
+                         a);
       break;
   }
   return hipGetLastError();
@@ -164,7 +171,7 @@ hipError_t launch_bandwidth_probe(int kind, int lattice, const void* src, void* 
   const long long rows = D == 3 ? (long long)nx * ny : nx;
   long long waves = rows * a.segs;
   if (kind == LBM_PROBE_COPY) waves = (a.pop_bytes * Q + (long long)Q * 1024 - 1) / ((long long)Q * 1024);
-  const int grid = (int)((waves + 3) / 4);
+  const int grid = (int)((waves + kVecWaves - 1) / kVecWaves);
   switch (lattice) {
     case (int)LatticeType::D2Q9:
       return launch_for<Lattice<LatticeType::D2Q9>>(kind, a, grid, st);

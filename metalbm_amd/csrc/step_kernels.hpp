@@ -41,6 +41,7 @@ struct StepArgs {
   double guo_coef;      // 1 - 1 / (2 tau0)      (ForcingScheme.h:115)
   const void* ftab[3];  // per-component force tables (compute type) or NULL
   int fax[3];           // axis indexing table d: 0 x, 1 y, 2 z, 3 constant, 4 cell
+  int nt_loads;         // BGK cell kernel: non-temporal population loads
   int store;            // write output fields on this step
   double* rho_out;
   double* uh_out;       // [D][N]
@@ -76,15 +77,30 @@ enum : int { TYPE_F64 = 0, TYPE_F32 = 1, TYPE_F16 = 2 };
 // zero collision source (ForcingScheme.h:68-78, 141-151).
 enum : int { SRC_ZERO = 0, SRC_GUO = 2, SRC_EDM = 4 };
 
+// Wavefronts per workgroup of the vector kernel (k_bgk_vec).  One: the
+// pattern of Q read and Q write streams measured fastest with
+// single-wavefront groups (pull-copy probe at 256^3 FP32: 5.45 TB/s at one
+// wavefront, 5.36 at two, 5.22 at four, 5.21 at eight).
+constexpr int kVecWaves = 1;
+
 __device__ __forceinline__ int xcd_tile(int b, int G) {
   // Workgroups are handed to the 8 XCDs round-robin (XCD = b mod 8).
   if ((G & 7) != 0) return b;
   return (b & 7) * (G >> 3) + (b >> 3);
 }
 
-template <class S>
+// Population loads, non-temporal when NT.  NT pays where one workgroup
+// covers a whole row and the lattices far exceed the caches (D3Q19 256^3
+// FP64: +5.5%); where several workgroups share a row, the lines the +-1
+// shifts along the row share between them are better kept (D3Q27 384^3 with
+// two tiles per row: -7% FP16 storage, -5% FP32 with non-temporal loads;
+// A/B on one MI355X).
+template <bool NT, class S>
 __device__ __forceinline__ S ld(const S* p) {
-  return *p;
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
 }
 
 template <class L, int i, class S, class C>
@@ -196,14 +212,14 @@ __device__ __forceinline__ void force_of_cell(const StepArgs& a,
 }
 
 // Pull all Q populations of the cell into registers.
-template <class L, class S, class C>
+template <class L, class S, class C, bool NT = false>
 __device__ __forceinline__ void pull(const StepArgs& a, const CellPos& c,
                                      C* f) {
   const S* __restrict__ src = static_cast<const S*>(a.src);
   static_for<L::dimQ>([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     const S* base = src + (size_t)i * (size_t)a.N;
-    f[i] = to_compute<L, i, S, C>(ld(base + pull_index<L, i>(a, c)));
+    f[i] = to_compute<L, i, S, C>(ld<NT>(base + pull_index<L, i>(a, c)));
   });
 }
 

@@ -338,10 +338,18 @@ int64_t lbm_steps_done(const lbm_handle *h);
  * libhdf5.so, libhdf5.so.103, /opt/conda/lib); LBM_E_STATE if absent. */
 /* Fields of the last storing step, in FieldList::writeFields order
  * (FieldList.h:65-86, Routine.h:202-210, Writer.h:309-360): density,
- * velocity{X,Y[,Z]}, alpha (ELBM only), force{X,Y[,Z]} (when a force is
- * configured).  Writes the XDMF sidecar of FieldWriter<XDMF>
- * (Writer.h:451-560) to xdmf_path unless it is NULL. */
+ * velocity{X,Y[,Z]}, then as lbm_set_field_output selects: alpha (2 for BGK,
+ * Collision.h:114-123), force{X,Y[,Z]}, and the vorticity of
+ * Curl::executeSpace + normalize (Routine.h:129-132; lbm_vorticity with
+ * LBM_SPECTRAL_REFERENCE): `vorticity` in 2D, vorticity{X,Y,Z} in 3D.
+ * Writes the XDMF sidecar of FieldWriter<XDMF> (Writer.h:451-560) to
+ * xdmf_path unless it is NULL. */
 int lbm_write_fields_hdf5(lbm_handle *h, const char *path, const char *xdmf_path);
+/* writeAlpha, writeForce, writeVorticity of the reference's input file
+ * (src/Input_prod.in:68-72): which optional datasets lbm_write_fields_hdf5
+ * writes.  All three are on for a new handle, as in every shipped input. */
+int lbm_set_field_output(lbm_handle *h, int write_alpha, int write_force,
+                         int write_vorticity);
 /* Unshifted FP64 populations as datasets distribution0..distribution{Q-1}
  * (DistributionWriter, Writer.h:385-440, Routine.h:212-216). */
 int lbm_write_distribution_hdf5(lbm_handle *h, const char *path);

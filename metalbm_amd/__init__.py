@@ -386,6 +386,12 @@ class Algorithm:
     # -- HDF5 files in the reference layout -------------------------------------
     # File names follow Writer::getFileName (Writer.h:44-50): <folder>/field-<it>.h5
     # with its field-<it>.xmf sidecar, and <folder>/distribution-<it>.h5.
+    def set_field_output(self, alpha: bool = True, force: bool = True,
+                         vorticity: bool = True) -> None:
+        """writeAlpha / writeForce / writeVorticity of the field files
+        (Input_prod.in:68-72; all on by default)."""
+        self._check(lib().lbm_set_field_output(self._h, int(alpha), int(force), int(vorticity)))
+
     def write_fields_hdf5(self, path: str, xdmf_path: str | None = None) -> None:
         """Fields of the last storing step as FP64 datasets (density,
         velocityX.., alpha for ELBM, forceX.. when forced)."""

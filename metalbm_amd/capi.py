@@ -28,7 +28,7 @@ EXPORTS = (
     "lbm_sample_log_count", "lbm_sample_log_copy", "lbm_mlp_train", "lbm_set_alpha_mode",
     "lbm_get_config", "lbm_write_fields_hdf5", "lbm_write_distribution_hdf5",
     "lbm_read_distribution_hdf5", "lbm_spectral_shells", "lbm_vorticity", "lbm_spectra",
-    "lbm_kernel_shape", "lbm_reduce_noalloc", "lbm_bandwidth_probe",
+    "lbm_kernel_shape", "lbm_reduce_noalloc", "lbm_bandwidth_probe", "lbm_set_field_output",
 )
 
 
@@ -130,6 +130,7 @@ def lib() -> ctypes.CDLL:
     L.lbm_kernel_shape.argtypes = [ctypes.POINTER(LbmConfigC), ip, ip]
     cp = ctypes.c_char_p
     L.lbm_write_fields_hdf5.argtypes = [H, cp, cp]
+    L.lbm_set_field_output.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.lbm_write_distribution_hdf5.argtypes = [H, cp]
     L.lbm_read_distribution_hdf5.argtypes = [H, cp]
     L.lbm_spectral_shells.argtypes = [H, ip]

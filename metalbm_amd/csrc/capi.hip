@@ -94,6 +94,9 @@ struct lbm_handle {
   // alpha modes 1 and 2: count and list of the cells the correction defers
   unsigned* defer_count = nullptr;
   int* defer_list = nullptr;
+  // datasets of the field file besides density and velocity
+  // (writeAlpha, writeForce, writeVorticity; all on in Input_prod.in:68-72)
+  int out_alpha = 1, out_force = 1, out_vorticity = 1;
 };
 
 namespace {
@@ -497,6 +500,12 @@ int lbm::reset_alpha(lbm_handle* h) {
   return LBM_OK;
 }
 
+void lbm::field_output(const lbm_handle* h, int* alpha, int* force, int* vorticity) {
+  *alpha = h->out_alpha;
+  *force = h->out_force;
+  *vorticity = h->out_vorticity;
+}
+
 int lbm::device_fields(lbm_handle* h, DeviceFields& out) {
   if (!h->fields_valid)
     return fail(h, LBM_E_STATE, "no storing step has run (lbm_step store_last)");
@@ -573,6 +582,14 @@ int lbm_get_config(const lbm_handle* h, lbm_config* out) {
   out->alpha_mode = h->alpha_mode;
   out->residual_tol = h->residual_tol;
   out->kernel = h->vec ? LBM_KERNEL_VECTOR : LBM_KERNEL_CELL;
+  return LBM_OK;
+}
+
+int lbm_set_field_output(lbm_handle* h, int write_alpha, int write_force, int write_vorticity) {
+  if (!h) return LBM_E_INVALID;
+  h->out_alpha = write_alpha ? 1 : 0;
+  h->out_force = write_force ? 1 : 0;
+  h->out_vorticity = write_vorticity ? 1 : 0;
   return LBM_OK;
 }
 

@@ -16,6 +16,9 @@ int record_error(lbm_handle* h, int code, const std::string& msg);
 // LBM_E_HIP with the HIP error text, unless e is hipSuccess (then LBM_OK).
 int record_hip(lbm_handle* h, hipError_t e, const char* what);
 
+// Datasets the field file writes besides density and velocity.
+void field_output(const lbm_handle* h, int* alpha, int* force, int* vorticity);
+
 // Sets the entropic alpha field to 2 (initAlpha, Initialize.h:81-88); no-op
 // for BGK.
 int reset_alpha(lbm_handle* h);

@@ -107,7 +107,6 @@ const H5& hdf5() {
 
 struct Grid {
   int D = 0, Q = 0, n[3] = {1, 1, 1};
-  bool elbm = false, forced = false;
   int64_t N = 0;
   hsize_t dims[3] = {1, 1, 1};  // padded, as gSD::pLength()
   size_t padded = 0;
@@ -120,8 +119,6 @@ int grid_of(lbm_handle* h, Grid& g) {
   g.n[0] = c.nx;
   g.n[1] = c.ny;
   g.n[2] = c.nz;
-  g.elbm = c.collision == LBM_ELBM;
-  g.forced = c.force != LBM_FORCE_NONE;
   g.padded = 1;
   for (int d = 0; d < g.D; ++d) {
     g.dims[d] = (hsize_t)(d == g.D - 1 ? 2 * (g.n[d] / 2 + 1) : g.n[d]);
@@ -247,17 +244,32 @@ int lbm_write_fields_hdf5(lbm_handle* h, const char* path, const char* xdmf_path
   if (int rc = lbm_get_fields(h, rho.data(), u.data(), alpha.data(), F.data())) return rc;
   File f(x, x.fcreate(path, kAccTrunc, kDefault, kDefault));
   if (f.id() < 0) return lbm::record_error(h, LBM_E_INVALID, std::string("cannot create ") + path);
+  int w_alpha = 1, w_force = 1, w_vort = 1;
+  lbm::field_output(h, &w_alpha, &w_force, &w_vort);
+  // vorticity as the reference stores it: curl of u_h by Curl::executeSpace +
+  // normalize (Routine.h:129-132), one component in 2D, three in 3D
+  const int C = 2 * g.D - 3;
+  std::vector<double> omega;
+  if (w_vort) {
+    omega.resize((size_t)C * g.N);
+    if (int rc = lbm_vorticity(h, LBM_SPECTRAL_REFERENCE, omega.data(), nullptr)) return rc;
+  }
   std::vector<std::string> names;
   auto put = [&](const std::string& name, const double* data) {
     names.push_back(name);
     return write_dataset(x, f.id(), g, name, data);
   };
+  // FieldList::writeFields order (FieldList.h:65-86); a field of several
+  // components gets the axis suffix per component (Writer.h:317-320)
   bool ok = put("density", rho.data());
   for (int d = 0; ok && d < g.D; ++d)
     ok = put(std::string("velocity") + kAxis[d], u.data() + (size_t)d * g.N);
-  if (ok && g.elbm) ok = put("alpha", alpha.data());
-  for (int d = 0; ok && g.forced && d < g.D; ++d)
+  if (ok && w_alpha) ok = put("alpha", alpha.data());
+  for (int d = 0; ok && w_force && d < g.D; ++d)
     ok = put(std::string("force") + kAxis[d], F.data() + (size_t)d * g.N);
+  for (int k = 0; ok && w_vort && k < C; ++k)
+    ok = put(C == 1 ? std::string("vorticity") : std::string("vorticity") + kAxis[k],
+             omega.data() + (size_t)k * g.N);
   if (!ok) return lbm::record_error(h, LBM_E_INVALID, std::string("writing ") + path + " failed");
   if (xdmf_path) write_xdmf(g, xdmf_path, base_name(path), names);
   return LBM_OK;

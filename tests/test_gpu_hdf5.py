@@ -2,7 +2,8 @@
 
 The reference writes FP64 datasets of dimensions gSD::pLength() -- the last
 extent padded to 2 (n/2 + 1) (Domain.h:53-57, MathVector.h:331-344) -- named
-density, velocityX.., alpha, forceX.. (FieldList.h:65-86, Writer.h:309-360)
+density, velocityX.., alpha, forceX.., vorticity (FieldList.h:65-86,
+Writer.h:309-360)
 and distribution0..Q-1 (Writer.h:385-440), and restarts by reading the
 distribution back (Reader.h:106-158, Initialize.h:119-145).  The files are
 read here by an independent ctypes reader (tests/h5util.py) and by h5dump."""
@@ -47,9 +48,19 @@ CASES = [
 ]
 
 
+OUTPUTS = [(True, True, True), (False, False, False), (True, False, True)]
+
+
+@pytest.mark.parametrize("out", OUTPUTS, ids=["alpha-force-vorticity", "none", "alpha-vorticity"])
 @pytest.mark.parametrize("name,c,shape", CASES, ids=[x[0] for x in CASES])
-def test_fields_file_layout(tmp_path, name, c, shape):
-    print(f"case hdf5 fields {name}", flush=True)
+def test_fields_file_layout(tmp_path, name, c, shape, out):
+    """writeAlpha, writeForce and writeVorticity select the optional datasets
+    (all on in every shipped input, Input_prod.in:68-72): alpha (2 for BGK),
+    the force components (zero without a force) and the reference-scaled
+    vorticity (Curl::executeSpace + normalize, Routine.h:129-132): one
+    dataset `vorticity` in 2D, vorticityX/Y/Z in 3D (FieldList.h:40, 84)."""
+    print(f"case hdf5 fields {name} {out}", flush=True)
+    w_alpha, w_force, w_vort = out
     with Algorithm(c) as a:
         a.init_noise(1.0, 2e-2, 5)
         with pytest.raises(LBMError) as e:
@@ -57,18 +68,29 @@ def test_fields_file_layout(tmp_path, name, c, shape):
         assert e.value.code == LBM_E_STATE
         a.iterate(3, store=True)
         rho, u, al, F = a.store_fields()
+        om = a.vorticity("reference")[0]
+        if out != (True, True, True):
+            a.set_field_output(*out)
         path = a.write_fields(str(tmp_path), 3)
     assert os.path.basename(path) == "field-3.h5"
     D = len(shape)
     names = ["density"] + [f"velocity{AX[d]}" for d in range(D)]
     expect = [rho] + [u[d] for d in range(D)]
-    if c.collision == CollisionType.ELBM:
+    if c.collision == CollisionType.BGK:
+        assert np.all(al == 2.0)  # the BGK collision's alpha (Collision.h:114-123)
+    if c.force == ForceType.None_:
+        assert not F.any()
+    if w_alpha:
         names.append("alpha")
         expect.append(al)
-    if c.force != ForceType.None_:
+    if w_force:
         names += [f"force{AX[d]}" for d in range(D)]
         expect += [F[d] for d in range(D)]
-    absent = {"alpha", *[f"force{AX[d]}" for d in range(D)]} - set(names)
+    vnames = ["vorticity"] if D == 2 else [f"vorticity{AX[d]}" for d in range(3)]
+    if w_vort:
+        names += vnames
+        expect += [om[k] for k in range(len(vnames))]
+    absent = {"alpha", *[f"force{AX[d]}" for d in range(D)], *vnames} - set(names)
     with h5util.H5File(path) as h:
         for n, v in zip(names, expect):
             arr = h.read(n)

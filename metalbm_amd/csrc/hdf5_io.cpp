@@ -25,12 +25,18 @@
 #include <vector>
 
 #include "capi_internal.hpp"
+#include "hdf5_layout.hpp"
 
 namespace {
 
+using lbm::Grid;
+using lbm::from_padded;
+using lbm::hsize_t;
+using lbm::set_extents;
+using lbm::to_padded;
+
 typedef int64_t hid_t;
 typedef int herr_t;
-typedef unsigned long long hsize_t;
 constexpr hid_t kDefault = 0;           // H5P_DEFAULT, H5E_DEFAULT
 constexpr unsigned kAccTrunc = 0x0002u;  // H5F_ACC_TRUNC
 constexpr unsigned kAccRdonly = 0x0000u;  // H5F_ACC_RDONLY
@@ -105,42 +111,14 @@ const H5& hdf5() {
   return h;
 }
 
-struct Grid {
-  int D = 0, Q = 0, n[3] = {1, 1, 1};
-  int64_t N = 0;
-  hsize_t dims[3] = {1, 1, 1};  // padded, as gSD::pLength()
-  size_t padded = 0;
-};
-
 int grid_of(lbm_handle* h, Grid& g) {
   lbm_config c;
   if (int rc = lbm_get_config(h, &c)) return rc;
-  if (int rc = lbm_sizes(h, &g.N, &g.Q, &g.D)) return rc;
-  g.n[0] = c.nx;
-  g.n[1] = c.ny;
-  g.n[2] = c.nz;
-  g.padded = 1;
-  for (int d = 0; d < g.D; ++d) {
-    g.dims[d] = (hsize_t)(d == g.D - 1 ? 2 * (g.n[d] / 2 + 1) : g.n[d]);
-    g.padded *= (size_t)g.dims[d];
-  }
+  int64_t N = 0;
+  int Q = 0, D = 0;
+  if (int rc = lbm_sizes(h, &N, &Q, &D)) return rc;
+  set_extents(g, D, Q, c.nx, c.ny, c.nz);
   return LBM_OK;
-}
-
-// Dense [x][y][z] (z fastest) into the padded layout and back.
-void to_padded(const Grid& g, const double* src, std::vector<double>& dst) {
-  dst.assign(g.padded, 0.0);
-  const int64_t rows = g.N / g.n[g.D - 1], len = g.n[g.D - 1];
-  const size_t plen = (size_t)g.dims[g.D - 1];
-  for (int64_t r = 0; r < rows; ++r)
-    for (int64_t k = 0; k < len; ++k) dst[(size_t)r * plen + k] = src[r * len + k];
-}
-
-void from_padded(const Grid& g, const std::vector<double>& src, double* dst) {
-  const int64_t rows = g.N / g.n[g.D - 1], len = g.n[g.D - 1];
-  const size_t plen = (size_t)g.dims[g.D - 1];
-  for (int64_t r = 0; r < rows; ++r)
-    for (int64_t k = 0; k < len; ++k) dst[r * len + k] = src[(size_t)r * plen + k];
 }
 
 class File {

@@ -234,6 +234,12 @@ __device__ __forceinline__ double newton_reference(const V& p, double guess,
   return x;
 }
 
+// Residual solvers (modes 1, 2, 4) evaluate G in the reference's log form
+// for D3Q27 (register budget) and in the log1p-series form for D2Q9 (cheaper
+// where every |alpha x_i| is small).
+template <class L>
+constexpr bool kLogResidual = L::dimQ > 9;
+
 __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
   unsigned h = cell * 0x9E3779B1u ^ salt;
   h ^= h >> 16;
@@ -252,8 +258,10 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // inference runs with every lane present: (1) pull, moments, fneq and the
 // alpha prelude per cell; (2) MODE 1 only: the network predicts alpha for
 // every lane's cell at once on the matrix cores; (3) the alpha solve, the
-// collision and the stores per cell.  Each wavefront copies the network into
-// its own LDS region before phase 1, so no workgroup barrier is involved.
+// collision and the stores per cell.  The workgroup stages the network in
+// LDS once, before phase 1, and one barrier before phase 2 publishes it (a
+// copy per wavefront, as first built, needed four times the LDS and held a
+// D3Q27 workgroup pair at the 160 KiB limit).
 // Minimum wavefronts per SIMD the register allocator must leave room for
 // (the FP64 solve is latency-bound, so occupancy beats a few spills; measured
 // on one MI355X): D2Q9 learned three (168 VGPRs, 2 spilled; 17.3 k MLUPS vs
@@ -284,12 +292,13 @@ __global__ void __launch_bounds__(BLOCK)
   extern __shared__ __align__(16) char s_lds[];
   if (a.stats && threadIdx.x < 6) s_cnt[threadIdx.x] = 0u;
   static_assert(MODE != 1 || Shape::NIN > 0, "no alpha network for this lattice");
-  // MODE 1: each wavefront copies the packed network into its own LDS region
-  // now; the copy completes while phase 1 runs.
+  // MODE 1: the workgroup copies the packed network into its LDS now; the
+  // copy completes while phase 1 runs, and one barrier before phase 2
+  // publishes it.
   using Lay = MfmaLayout<Shape::NIN, Shape::H1, Shape::H2>;
-  char* wave_lds = s_lds + (threadIdx.x >> 6) * Lay::WAVE;
+  char* wave_tile = s_lds + Lay::NET + (threadIdx.x >> 6) * Lay::TILE;
   if constexpr (MODE == 1)
-    copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, wave_lds);
+    copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, s_lds);
   // MODE 3: the reference Newton for the cells the correction of MODE 1 or 2
   // deferred (one thread per listed cell).
   CellPos c;
@@ -355,7 +364,9 @@ __global__ void __launch_bounds__(BLOCK)
 #pragma unroll
       for (int k = 0; k < Shape::NIN; ++k) m[k] = 0.0f;
     }
-    start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2, typename Shape::Op>(wave_lds, m);
+    __syncthreads();  // the staged network (every thread of the block reaches here)
+    start = mlp_alpha_wave<Shape::NIN, Shape::H1, Shape::H2, typename Shape::Op>(s_lds,
+                                                                                wave_tile, m);
   }
   // ---- phase 3 ----
   if (c.active) {
@@ -366,7 +377,14 @@ __global__ void __launch_bounds__(BLOCK)
       if constexpr (MODE == 0 || MODE == 3) {
         alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, its);
       } else if constexpr (MODE == 4) {
-        alpha = newton_residual<L>(p, amax, double(astate[cell]), a.residual_tol, branch, its);
+        if constexpr (kLogResidual<L>) {
+          branch = residual_newton_log<L, false>(p, amax, double(astate[cell]),
+                                                 a.residual_tol, 50, alpha, its) ? 2 : 3;
+          if (branch == 3) alpha = 2.0;
+        } else {
+          alpha = newton_residual<L>(p, amax, double(astate[cell]), a.residual_tol, branch,
+                                     its);
+        }
       } else {
         // MODE 1 starts from the network's alpha, MODE 2 from the stored one;
         // one certified correction, else the reference Newton decides -- in
@@ -374,7 +392,12 @@ __global__ void __launch_bounds__(BLOCK)
         if constexpr (MODE == 2) start = double(astate[cell]);
         int ev = 0;
         its = ev;
-        if (residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev)) {
+        bool accepted;
+        if constexpr (kLogResidual<L>)
+          accepted = residual_newton_log<L, true>(p, amax, start, a.residual_tol, 2, alpha, ev);
+        else
+          accepted = residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev);
+        if (accepted) {
           branch = 2;
           first = true;
         } else {

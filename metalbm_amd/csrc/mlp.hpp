@@ -87,25 +87,33 @@ __device__ __forceinline__ void nonequilibrium_moments(const T* fneq, T rrho,
 // dimension, so a layer's accumulator (lane l: cell l%16 of the tile, rows
 // 4(l/16)+j) is already the next layer's B operand (lane l: column l%16,
 // k = 4(l/16)+j): H^T = relu(W . X^T + b) chains without a transpose.
-// Every wavefront owns an LDS region of WAVE bytes (so no workgroup barrier
-// is needed): the packed network [0, NET), then its feature and output tiles.
-//   network: W1h[H1][KP] W2h[H2][H1] W3h[16][H2] (operand type, zero padded;
-//            W3h row 0 = W3), b1[H1] b2[H2] b3 mean[NIN] scale[NIN] out[2] (f32)
-//   X[64][KP] (standardised features of the wavefront's cells), Y[64] (f32)
+// A workgroup holds one copy of the packed network in LDS, [0, NET), staged
+// by all its threads before one barrier; every wavefront then owns a tile
+// region of TILE bytes at NET + wave TILE for its features and outputs.
+//   network: W1h[H1][KP+4] W2h[H2][H1+4] W3h[16][H2+4] (operand type, zero
+//            padded; W3h row 0 = W3), b1[H1] b2[H2] b3 mean[NIN] scale[NIN]
+//            out[2] (f32)
+//   X[64][KP+4] (standardised features of the wavefront's cells), Y[64] (f32)
 template <int NIN, int H1, int H2>
 struct MfmaLayout {
   static constexpr int KP = NIN <= 16 ? 16 : 32;
+  // Row strides in operand elements, padded by four (two dwords) so that the
+  // 16 rows one lane group reads (8 bytes each) fall in distinct LDS banks:
+  // unpadded, a stride of 16 or 32 dwords put every row on the same bank
+  // pair (measured: 1.6e9 bank-conflict cycles per D3Q27 256^3 step).
+  static constexpr int KPS = KP + 4, H1S = H1 + 4, H2S = H2 + 4;
   static constexpr int W1H = 0;
-  static constexpr int W2H = W1H + H1 * KP * 2;
-  static constexpr int W3H = W2H + H2 * H1 * 2;
-  static constexpr int F32 = W3H + 16 * H2 * 2;
+  static constexpr int W2H = W1H + H1 * KPS * 2;
+  static constexpr int W3H = W2H + H2 * H1S * 2;
+  static constexpr int F32 = W3H + 16 * H2S * 2;
   static constexpr int B1 = F32, B2 = B1 + 4 * H1, B3 = B2 + 4 * H2, MEAN = B3 + 4,
                        SCALE = MEAN + 4 * NIN, OUT = SCALE + 4 * NIN;
   static constexpr int NET = ((OUT + 8 + 15) / 16) * 16;
-  static constexpr int X = NET;
-  static constexpr int Y = X + 64 * KP * 2;
-  static constexpr int WAVE = ((Y + 64 * 4 + 15) / 16) * 16;
-  static constexpr int BYTES = 4 * WAVE;  // a workgroup of four wavefronts
+  // offsets inside a wavefront's tile region
+  static constexpr int X = 0;
+  static constexpr int Y = X + 64 * KPS * 2;
+  static constexpr int TILE = ((Y + 64 * 4 + 15) / 16) * 16;
+  static constexpr int BYTES = NET + 4 * TILE;  // a workgroup of four wavefronts
 };
 
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
@@ -141,12 +149,15 @@ __device__ __forceinline__ void stage_mfma_network(const float* __restrict__ P,
   T* w2 = reinterpret_cast<T*>(lds + M::W2H);
   T* w3 = reinterpret_cast<T*>(lds + M::W3H);
   float* fl = reinterpret_cast<float*>(lds + M::F32);
-  for (int t = threadIdx.x; t < H1 * M::KP; t += blockDim.x) {
-    const int r = t / M::KP, k = t - r * M::KP;
+  for (int t = threadIdx.x; t < H1 * M::KPS; t += blockDim.x) {
+    const int r = t / M::KPS, k = t - r * M::KPS;
     w1[t] = k < NIN ? T(P[Dm::W1 + r * NIN + k]) : T(0.0f);
   }
-  for (int t = threadIdx.x; t < H2 * H1; t += blockDim.x) w2[t] = T(P[Dm::W2 + t]);
-  for (int t = threadIdx.x; t < 16 * H2; t += blockDim.x)
+  for (int t = threadIdx.x; t < H2 * M::H1S; t += blockDim.x) {
+    const int r = t / M::H1S, k = t - r * M::H1S;
+    w2[t] = k < H1 ? T(P[Dm::W2 + r * H1 + k]) : T(0.0f);
+  }
+  for (int t = threadIdx.x; t < 16 * M::H2S; t += blockDim.x)
     w3[t] = t < H2 ? T(P[Dm::W3 + t]) : T(0.0f);
   for (int t = threadIdx.x; t < H1; t += blockDim.x) fl[t] = P[Dm::B1 + t];
   for (int t = threadIdx.x; t < H2; t += blockDim.x) fl[H1 + t] = P[Dm::B2 + t];
@@ -154,15 +165,16 @@ __device__ __forceinline__ void stage_mfma_network(const float* __restrict__ P,
     fl[H1 + H2 + t] = P[Dm::B3 + t];  // b3, mean[NIN], scale[NIN], out[2]
 }
 
-// Copy of the packed network (M::NET bytes, 16-aligned) into the calling
-// wavefront's LDS region by its own 64 lanes.
+// Copy of the packed network (M::NET bytes, 16-aligned) into the
+// workgroup's LDS by all its threads; the caller makes it visible with a
+// workgroup barrier before the first mlp_alpha_wave.
 template <int NIN, int H1, int H2>
 __device__ __forceinline__ void copy_packed_network(const void* __restrict__ src,
-                                                    char* wave_lds) {
+                                                    char* lds) {
   constexpr int n = MfmaLayout<NIN, H1, H2>::NET / 16;
   const uint4* s = static_cast<const uint4*>(src);
-  uint4* d = reinterpret_cast<uint4*>(wave_lds);
-  for (int t = threadIdx.x & 63; t < n; t += 64) d[t] = s[t];
+  uint4* d = reinterpret_cast<uint4*>(lds);
+  for (int t = threadIdx.x; t < n; t += blockDim.x) d[t] = s[t];
 }
 
 // Orders one wavefront's LDS writes before its later LDS reads by other lanes
@@ -175,24 +187,23 @@ __device__ __forceinline__ void wave_sync_lds() {
 }
 
 // alpha_hat for every lane's cell (m: raw features; lanes without a cell pass
-// zeros), from the network the wavefront copied into its own LDS region
-// (copy_packed_network).  The region is private to the wavefront, so it
+// zeros), from the workgroup's staged network `net` (copy_packed_network and
+// a barrier) and the wavefront's own tile region `tile`, which it
 // synchronises with itself only; all 64 lanes of the wavefront must be active.
 template <int NIN, int H1, int H2, class T>
-__device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
+__device__ __forceinline__ double mlp_alpha_wave(const char* net, char* tile, const float* m) {
   using M = MfmaLayout<NIN, H1, H2>;
   using V4 = typename Mfma<T>::v4;
-  constexpr int KP = M::KP;
+  constexpr int KP = M::KP, KPS = M::KPS, H1S = M::H1S, H2S = M::H2S;
   const int lane = threadIdx.x & 63;
-  wave_sync_lds();  // the wavefront's network copy is complete
-  const float* fl = reinterpret_cast<const float*>(lds + M::F32);
+  const float* fl = reinterpret_cast<const float*>(net + M::F32);
   const float* b1 = fl;
   const float* b2 = fl + H1;
   const float b3 = fl[H1 + H2];
   const float* mean = fl + H1 + H2 + 1;
   const float* scale = mean + NIN;
-  T* X = reinterpret_cast<T*>(lds + M::X);
-  float* Y = reinterpret_cast<float*>(lds + M::Y);
+  T* X = reinterpret_cast<T*>(tile + M::X);
+  float* Y = reinterpret_cast<float*>(tile + M::Y);
 #pragma unroll
   for (int k4 = 0; k4 < KP; k4 += 4) {
     V4 z;
@@ -201,12 +212,12 @@ __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
       const int k = k4 + j;
       z[j] = k < NIN ? T((m[k] - mean[k]) * scale[k]) : T(0.0f);
     }
-    *reinterpret_cast<V4*>(X + lane * KP + k4) = z;
+    *reinterpret_cast<V4*>(X + lane * KPS + k4) = z;
   }
   wave_sync_lds();
-  const T* w1 = reinterpret_cast<const T*>(lds + M::W1H);
-  const T* w2 = reinterpret_cast<const T*>(lds + M::W2H);
-  const T* w3 = reinterpret_cast<const T*>(lds + M::W3H);
+  const T* w1 = reinterpret_cast<const T*>(net + M::W1H);
+  const T* w2 = reinterpret_cast<const T*>(net + M::W2H);
+  const T* w3 = reinterpret_cast<const T*>(net + M::W3H);
   const int col = lane & 15, kq = 4 * (lane >> 4);
   auto frag = [](const T* p) { return *reinterpret_cast<const V4*>(p); };
 #pragma unroll
@@ -217,8 +228,8 @@ __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
       mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kb = 0; kb < KP / 16; ++kb)
-        acc = Mfma<T>::mac(frag(w1 + (rt * 16 + col) * KP + kb * 16 + kq),
-                           frag(X + (t * 16 + col) * KP + kb * 16 + kq), acc);
+        acc = Mfma<T>::mac(frag(w1 + (rt * 16 + col) * KPS + kb * 16 + kq),
+                           frag(X + (t * 16 + col) * KPS + kb * 16 + kq), acc);
 #pragma unroll
       for (int j = 0; j < 4; ++j) h1[rt][j] = T(fmaxf(acc[j] + b1[rt * 16 + kq + j], 0.0f));
     }
@@ -228,14 +239,14 @@ __device__ __forceinline__ double mlp_alpha_wave(char* lds, const float* m) {
       mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kb = 0; kb < H1 / 16; ++kb)
-        acc = Mfma<T>::mac(frag(w2 + (rt * 16 + col) * H1 + kb * 16 + kq), h1[kb], acc);
+        acc = Mfma<T>::mac(frag(w2 + (rt * 16 + col) * H1S + kb * 16 + kq), h1[kb], acc);
 #pragma unroll
       for (int j = 0; j < 4; ++j) h2[rt][j] = T(fmaxf(acc[j] + b2[rt * 16 + kq + j], 0.0f));
     }
     mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kb = 0; kb < H2 / 16; ++kb)
-      acc = Mfma<T>::mac(frag(w3 + col * H2 + kb * 16 + kq), h2[kb], acc);
+      acc = Mfma<T>::mac(frag(w3 + col * H2S + kb * 16 + kq), h2[kb], acc);
     if (lane < 16) Y[t * 16 + lane] = acc[0];  // row 0 = the output unit
   }
   wave_sync_lds();
@@ -431,6 +442,81 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
   if (!(bound <= tol)) return false;
   alpha = a1;
   return true;
+}
+
+// Residual Newton with G and G' in the reference's logarithmic form
+// (EntropicStep.h:31-62): G(a) = sum f ln(f/w) - sum (f - a n) ln((f - a n)/w),
+// G'(a) = sum n (1 + ln((f - a n)/w)), one library logarithm per term, the
+// constant sum taken once.  Up to max_evals evaluations from `start`:
+// accepts the first iterate with |G| <= tol inside (1, amax) -- if STRICT,
+// also a Newton point whose Taylor remainder bound is <= tol, without
+// evaluating G there (see below); returns false
+// when an iterate leaves the evaluation interval -- (1, amax) if STRICT (the
+// one-correction solvers), else (0, amax), where every f - a n > 0 so every
+// logarithm is defined -- or max_evals evaluations do not reach tol.
+// FP64 rounding of G is ~1e-15 absolute here (terms of order 1e-2), far below
+// the tolerances used.  Fewer live values than the log1p form of
+// residual_terms (no x_i, no series variants, no curvature), which keeps the
+// D3Q27 kernels within their register budget; D2Q9 keeps the cheaper series
+// form.  evals: G evaluations made.
+template <class L, bool STRICT, class V>
+__device__ __forceinline__ bool residual_newton_log(const V& p, double amax, double start,
+                                                    double tol, int max_evals,
+                                                    double& alpha, int& evals) {
+  constexpr int Q = L::dimQ;
+  evals = 0;
+  double SA = 0.0;
+  static_for<Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr double inv_w = 1.0 / L::W[i];
+    SA += p.template f<i>() * log(p.template f<i>() * inv_w);
+  });
+  double a = start;
+  for (int it = 0; it < max_evals; ++it) {
+    if (!(a > (STRICT ? 1.0 : 0.0) && a < amax)) return false;
+    // STRICT: each evaluation also gathers the curvature terms of the
+    // certificate below, h = sum n^2 / fm and r = max |n| / fm, in the same
+    // pass over the populations.
+    constexpr bool cert = STRICT;
+    double g = SA, dg = 0.0, h = 0.0, r = 0.0;
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr double inv_w = 1.0 / L::W[i];
+      const double ni = p.template n<i>();
+      const double fm = p.template f<i>() - a * ni;
+      const double lg = log(fm * inv_w);
+      g -= fm * lg;
+      dg += ni * (1.0 + lg);
+      if constexpr (cert) {
+        const double rf = recip_refined(fm);
+        h += ni * ni * rf;
+        r = fmax(r, fabs(ni) * rf);
+      }
+    });
+    evals = it + 1;
+    if (fabs(g) <= tol) {
+      if (!(a > 1.0)) return false;
+      alpha = a;
+      return true;
+    }
+    const double step = g / dg;
+    if constexpr (cert) {
+      // Certificate of the Newton point a1 = a - step without evaluating G
+      // there: G(a1) = 1/2 step^2 G''(xi) for some xi between a and a1, and
+      // f - xi n = fm + (a - xi) n >= fm (1 - |step| r) > 0, so
+      // |G''(xi)| = sum n^2 / (f - xi n) <= h / (1 - |step| r); a relative
+      // allowance of 1e-6 covers the rounding of the bound.
+      const double a1 = a - step;
+      const double room = 1.0 - fabs(step) * r;
+      if (a1 > 1.0 && a1 < amax && room > 0.0 &&
+          0.5 * step * step * h / room * (1.0 + 1e-6) <= tol) {
+        alpha = a1;
+        return true;
+      }
+    }
+    a -= step;
+  }
+  return false;
 }
 
 // Plain Newton to the residual tolerance: from `start` (the stored alpha),

@@ -182,12 +182,14 @@ __global__ void __launch_bounds__(256)
 k_mlp_forward(const void* __restrict__ packed, const float* __restrict__ X, int n,
               double* __restrict__ alpha) {
   extern __shared__ __align__(16) char s_lds[];
-  char* wave_lds = s_lds + (threadIdx.x >> 6) * MfmaLayout<NIN, H1, H2>::WAVE;
-  copy_packed_network<NIN, H1, H2>(packed, wave_lds);
+  using M = MfmaLayout<NIN, H1, H2>;
+  char* wave_tile = s_lds + M::NET + (threadIdx.x >> 6) * M::TILE;
+  copy_packed_network<NIN, H1, H2>(packed, s_lds);
+  __syncthreads();
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   float m[NIN];
   for (int k = 0; k < NIN; ++k) m[k] = s < n ? X[(size_t)s * NIN + k] : 0.0f;
-  const double y = mlp_alpha_wave<NIN, H1, H2, T>(wave_lds, m);
+  const double y = mlp_alpha_wave<NIN, H1, H2, T>(s_lds, wave_tile, m);
   if (s < n) alpha[s] = y;
 }
 

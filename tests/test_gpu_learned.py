@@ -218,17 +218,17 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
     # tol; the network's start is accepted or certified at the first.
     assert per[AlphaMode.LEARNED] < 0.8 * per[AlphaMode.NEWTON_RESIDUAL]
     if lat == LatticeType.D3Q27:
-        # In the D3Q27 noise case the previous step's alpha is a poor start
-        # for about one cell in seven: one certified correction from it fails
-        # there (the cell then pays the reference Newton as well), from the
-        # network's start it succeeds.  (In the D2Q9 case every alpha stays
-        # within ~1e-3 of the root and the correction from any of these starts
-        # is certified.)
-        assert ev_l < 0.8 * ev_w
-        acc_l = st_l["learned_accepted"] / max(1, st_l["newton"])
-        acc_w = st_w["learned_accepted"] / max(1, st_w["newton"])
-        assert acc_l > acc_w + 0.1
+        # In the D3Q27 noise case the previous step's alpha (and alpha = 2) is
+        # a poorer start than the network's: its Newton point fails the
+        # certificate for about one cell in seven, which then pays a second
+        # evaluation; from the network's start one evaluation settles the
+        # cell.  (In the D2Q9 case every alpha stays within ~1e-3 of the root
+        # and the correction from any of these starts is certified.)
+        assert ev_l < 0.95 * ev_w
         assert per[AlphaMode.LEARNED] < per[AlphaMode.WARM_RESIDUAL]
+        # the network's start is accepted (first evaluation or its Newton
+        # point) for practically every solved cell
+        assert st_l["learned_accepted"] >= 0.99 * st_l["newton"]
 
 
 ALL_MODES = (AlphaMode.NEWTON, AlphaMode.NEWTON_RESIDUAL, AlphaMode.WARM_RESIDUAL,

@@ -490,8 +490,8 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
     """Energy and enstrophy of D3Q27 entropic decaying turbulence (n^3,
     tau = 0.50006, spectral initial field on shells 1..16 peaked at 4 with
     per-component rms 0.1 cs, seed 42) with FP32 compute and shifted FP32
-    storage -- reference Newton, certified correction of the stored alpha,
-    learned alpha -- against the FP64 reference-order build (equal to the FP64 oracle
+    storage -- reference Newton, plain Newton at the residual tolerance,
+    certified correction of the stored alpha, learned alpha -- against the FP64 reference-order build (equal to the FP64 oracle
     to <= 1e-12 on every tested configuration).  E and Omega every `every` steps
     (E = (1/N) sum 1/2 rho |u_h|^2, Omega = (1/N) sum 1/2 |curl u_h|^2 by
     central differences).  The learned run logs (features, alpha) over its
@@ -508,6 +508,7 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
     runs = {"f64-reference": (dict(storage=m.Precision.F64, compute=m.Precision.F64,
                                    shifted=False), m.AlphaMode.NEWTON),
             "f32-newton": (f32, m.AlphaMode.NEWTON),
+            "f32-newton-residual": (f32, m.AlphaMode.NEWTON_RESIDUAL),
             "f32-warm-residual": (f32, m.AlphaMode.WARM_RESIDUAL),
             "f32-learned": (f32, m.AlphaMode.LEARNED),
             "f32-learned-retrained": (f32, m.AlphaMode.LEARNED)}

@@ -89,7 +89,10 @@ enum {
                              second launch over the cells so deferred.  The
                              alpha state is kept as for NEWTON. */
   LBM_ALPHA_WARM_RESIDUAL = 2,  /* the same with the stored alpha as the start:
-                             the method without the network */
+                             the method without the network.  On D3Q27 it may
+                             take a second certified Newton step: from the
+                             stored alpha these are the reference Newton's own
+                             iterates, so the outcome stays the reference's */
   LBM_ALPHA_NEWTON_RESIDUAL = 3  /* plain Newton at the residual tolerance: the
                              reference Newton (EntropicStep.h:111-140) from the
                              stored alpha, stopped when |G(alpha)| <=

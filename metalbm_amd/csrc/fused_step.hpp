@@ -387,14 +387,21 @@ __global__ void __launch_bounds__(BLOCK)
         }
       } else {
         // MODE 1 starts from the network's alpha, MODE 2 from the stored one;
-        // one certified correction, else the reference Newton decides -- in
-        // the MODE 3 launch that follows, from the untouched stored alpha.
+        // if the solve below is not accepted the reference Newton decides --
+        // in the MODE 3 launch that follows, from the untouched stored alpha.
+        // MODE 1 makes one certified correction.  MODE 2 (D3Q27) may take a
+        // second: from the stored alpha its iterates are the reference
+        // Newton's own, so whatever it accepts is the root that Newton
+        // converges to.  From the network's alpha a second step would also
+        // reach roots from which the reference Newton, started at the stored
+        // alpha, diverges to its alpha = 2 fallback.
         if constexpr (MODE == 2) start = double(astate[cell]);
         int ev = 0;
         its = ev;
         bool accepted;
         if constexpr (kLogResidual<L>)
-          accepted = residual_newton_log<L, true>(p, amax, start, a.residual_tol, 2, alpha, ev);
+          accepted = residual_newton_log<L, true>(p, amax, start, a.residual_tol,
+                                                  MODE == 1 ? 1 : 2, alpha, ev);
         else
           accepted = residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev);
         if (accepted) {

@@ -452,7 +452,7 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
 // also a Newton point whose Taylor remainder bound is <= tol, without
 // evaluating G there (see below); returns false
 // when an iterate leaves the evaluation interval -- (1, amax) if STRICT (the
-// one-correction solvers), else (0, amax), where every f - a n > 0 so every
+// certified solvers), else (0, amax), where every f - a n > 0 so every
 // logarithm is defined -- or max_evals evaluations do not reach tol.
 // FP64 rounding of G is ~1e-15 absolute here (terms of order 1e-2), far below
 // the tolerances used.  Fewer live values than the log1p form of

@@ -134,6 +134,21 @@ struct Fp64View {
   __device__ __forceinline__ double n() const {
     return double(ns[i]);
   }
+  // The same values, converted afresh at each use: the empty asm statement
+  // on the stored value keeps the compiler from hoisting the conversions out
+  // of an iteration loop, which would hold 2Q doubles live across it.
+  template <int i>
+  __device__ __forceinline__ double f_fresh() const {
+    C v = fs[i];
+    asm volatile("" : "+v"(v));
+    return SH ? double(v) + L::W[i] : double(v);
+  }
+  template <int i>
+  __device__ __forceinline__ double n_fresh() const {
+    C v = ns[i];
+    asm volatile("" : "+v"(v));
+    return double(v);
+  }
 };
 
 // ELBM alpha (Collision.h:351-375), first part: the deviation test
@@ -267,6 +282,11 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // on one MI355X): D2Q9 learned three (168 VGPRs, 2 spilled; 17.3 k MLUPS vs
 // 13.8 k at two); D3Q27 two (reference Newton 2.72 k vs 2.03 k MLUPS at one
 // wavefront with 289 registers; certified correction +16%, learned +9%).
+// Workgroups of the MODE 3 launch: eight per CU, enough to keep every SIMD
+// busy at the entropic kernels' occupancy (two to four wavefronts per SIMD)
+// while the list is long.
+constexpr int kDeferBlocks = 2048;
+
 // Resets the deferred-solve counter before the correction launch (a kernel,
 // not a memset, so that the step can be captured in a graph).
 template <int UNUSED = 0>
@@ -300,11 +320,23 @@ __global__ void __launch_bounds__(BLOCK)
   if constexpr (MODE == 1)
     copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, s_lds);
   // MODE 3: the reference Newton for the cells the correction of MODE 1 or 2
-  // deferred (one thread per listed cell).
+  // deferred, one thread per listed cell, the workgroups striding over the
+  // list (a launch of at most kDeferBlocks workgroups: when nothing was
+  // deferred it costs little more than an empty launch).  Every other mode
+  // makes one pass.
+  const int n_listed = MODE == 3 ? (int)*a.defer_count : 0;
+  // MODE 3 tallies its statistics per thread over the passes (branches 0-3,
+  // Newton iterations, accepted corrections) and adds them up once at the
+  // end, so that the passes need no barrier.
+  unsigned tally[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+  for (int list_base = blockIdx.x * BLOCK;; list_base += gridDim.x * BLOCK) {
+  if constexpr (MODE == 3) {
+    if (list_base >= n_listed) break;  // uniform over the workgroup
+  }
   CellPos c;
   if constexpr (MODE == 3) {
-    const int t = blockIdx.x * BLOCK + threadIdx.x;
-    if (t < (int)*a.defer_count) {
+    const int t = list_base + threadIdx.x;
+    if (t < n_listed) {
       c = cell_from_index<L>(a, a.defer_list[t]);
     } else {
       c = CellPos{};
@@ -378,7 +410,7 @@ __global__ void __launch_bounds__(BLOCK)
         alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, its);
       } else if constexpr (MODE == 4) {
         if constexpr (kLogResidual<L>) {
-          branch = residual_newton_log<L, false>(p, amax, double(astate[cell]),
+          branch = residual_newton_log<L, false, false>(p, amax, double(astate[cell]),
                                                  a.residual_tol, 50, alpha, its) ? 2 : 3;
           if (branch == 3) alpha = 2.0;
         } else {
@@ -400,8 +432,8 @@ __global__ void __launch_bounds__(BLOCK)
         its = ev;
         bool accepted;
         if constexpr (kLogResidual<L>)
-          accepted = residual_newton_log<L, true>(p, amax, start, a.residual_tol,
-                                                  MODE == 1 ? 1 : 2, alpha, ev);
+          accepted = residual_newton_log<L, true, MODE == 2>(p, amax, start, a.residual_tol,
+                                                             MODE == 1 ? 1 : 2, alpha, ev);
         else
           accepted = residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev);
         if (accepted) {
@@ -488,12 +520,33 @@ __global__ void __launch_bounds__(BLOCK)
     }
   }
   if (a.stats) {
-    __syncthreads();
     // a deferred cell counts its correction's evaluations here and its
     // branch and Newton iterations in the MODE 3 launch
-    if (branch >= 0) atomicAdd(&s_cnt[branch], 1u);
-    if (its) atomicAdd(&s_cnt[4], (unsigned)its);
-    if (first) atomicAdd(&s_cnt[5], 1u);
+    if constexpr (MODE == 3) {
+      // static indices only: a runtime index would put the tally in scratch
+      static_for<4>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        tally[b] += branch == b ? 1u : 0u;
+      });
+      tally[4] += (unsigned)its;
+      tally[5] += first ? 1u : 0u;
+    } else {
+      __syncthreads();
+      if (branch >= 0) atomicAdd(&s_cnt[branch], 1u);
+      if (its) atomicAdd(&s_cnt[4], (unsigned)its);
+      if (first) atomicAdd(&s_cnt[5], 1u);
+    }
+  }
+  if constexpr (MODE != 3) break;
+  }  // list passes
+  if (a.stats) {
+    if constexpr (MODE == 3) {
+      __syncthreads();
+      static_for<6>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (tally[k]) atomicAdd(&s_cnt[k], tally[k]);
+      });
+    }
     __syncthreads();
     if (threadIdx.x < 6 && s_cnt[threadIdx.x])
       atomicAdd(&a.stats[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);

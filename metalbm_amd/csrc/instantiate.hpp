@@ -30,7 +30,7 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
                        dim3(block), 0, st, a, fsx);
   else {
     // Modes 1 and 2: the certified correction, then the reference Newton for
-    // the cells it deferred (MODE 3, one thread per possible list entry).
+    // the cells it deferred (MODE 3, striding over the list).
     hipLaunchKernelGGL((k_defer_reset<0>), dim3(1), dim3(64), 0, st, a.defer_count);
     if (a.alpha_mode == 2) {
       hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 2>), dim3(grid),
@@ -43,8 +43,9 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
     } else {
       return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>), dim3((a.N + 255) / 256),
-                       dim3(256), 0, st, a, fsx);
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>),
+                       dim3(std::min((a.N + 255) / 256, kDeferBlocks)), dim3(256), 0, st, a,
+                       fsx);
   }
   return hipGetLastError();
 }

@@ -458,8 +458,13 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
 // the tolerances used.  Fewer live values than the log1p form of
 // residual_terms (no x_i, no series variants, no curvature), which keeps the
 // D3Q27 kernels within their register budget; D2Q9 keeps the cheaper series
-// form.  evals: G evaluations made.
-template <class L, bool STRICT, class V>
+// form.  evals: G evaluations made.  FRESH converts the stored populations
+// to FP64 anew in every evaluation (V::f_fresh / n_fresh) instead of letting
+// the compiler keep 2Q converted doubles live across the iterations: with
+// the certificate's terms that held the two-evaluation D3Q27 solver at 61
+// spilled registers (7 with FRESH); the one-launch plain Newton, with fewer
+// live terms, runs faster without it.
+template <class L, bool STRICT, bool FRESH, class V>
 __device__ __forceinline__ bool residual_newton_log(const V& p, double amax, double start,
                                                     double tol, int max_evals,
                                                     double& alpha, int& evals) {
@@ -482,8 +487,15 @@ __device__ __forceinline__ bool residual_newton_log(const V& p, double amax, dou
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       constexpr double inv_w = 1.0 / L::W[i];
-      const double ni = p.template n<i>();
-      const double fm = p.template f<i>() - a * ni;
+      double ni, fi;
+      if constexpr (FRESH) {
+        ni = p.template n_fresh<i>();
+        fi = p.template f_fresh<i>();
+      } else {
+        ni = p.template n<i>();
+        fi = p.template f<i>();
+      }
+      const double fm = fi - a * ni;
       const double lg = log(fm * inv_w);
       g -= fm * lg;
       dg += ni * (1.0 + lg);

@@ -280,8 +280,10 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // Minimum wavefronts per SIMD the register allocator must leave room for
 // (the FP64 solve is latency-bound, so occupancy beats a few spills; measured
 // on one MI355X): D2Q9 learned three (168 VGPRs, 2 spilled; 17.3 k MLUPS vs
-// 13.8 k at two); D3Q27 two (reference Newton 2.72 k vs 2.03 k MLUPS at one
-// wavefront with 289 registers; certified correction +16%, learned +9%).
+// 13.8 k at two); D3Q27 learned three (168 VGPRs, none spilled; its launch
+// 3.81 -> 3.47 ms at 256^3 against two); the other D3Q27 solvers two
+// (reference Newton 2.72 k vs 2.03 k MLUPS at one wavefront with 289
+// registers; at three it spills 42 VGPRs).
 // Workgroups of the MODE 3 launch: eight per CU, enough to keep every SIMD
 // busy at the entropic kernels' occupancy (two to four wavefronts per SIMD)
 // while the list is long.
@@ -297,6 +299,7 @@ __global__ void k_defer_reset(unsigned* count) {
 template <class L, int MODE>
 constexpr int elbm_min_waves() {
   if (L::Type == LatticeType::D2Q9 && MODE == 1) return 3;
+  if (L::Type == LatticeType::D3Q27 && MODE == 1) return 3;
   if (L::Type == LatticeType::D3Q27) return 2;
   return 1;
 }
@@ -410,8 +413,11 @@ __global__ void __launch_bounds__(BLOCK)
         alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, its);
       } else if constexpr (MODE == 4) {
         if constexpr (kLogResidual<L>) {
+          EvalCurv cv;
           branch = residual_newton_log<L, false, false>(p, amax, double(astate[cell]),
-                                                 a.residual_tol, 50, alpha, its) ? 2 : 3;
+                                                        a.residual_tol, 50, alpha, its, cv)
+                       ? 2
+                       : 3;
           if (branch == 3) alpha = 2.0;
         } else {
           alpha = newton_residual<L>(p, amax, double(astate[cell]), a.residual_tol, branch,
@@ -431,11 +437,18 @@ __global__ void __launch_bounds__(BLOCK)
         int ev = 0;
         its = ev;
         bool accepted;
+        EvalCurv cv;
         if constexpr (kLogResidual<L>)
           accepted = residual_newton_log<L, true, MODE == 2>(p, amax, start, a.residual_tol,
-                                                             MODE == 1 ? 1 : 2, alpha, ev);
+                                                             MODE == 1 ? 1 : 2, alpha, ev, cv);
         else
-          accepted = residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev);
+          accepted = residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev, cv);
+        // The network's root stands only where the reference Newton from the
+        // stored alpha provably reaches it; else the cell is deferred and
+        // that Newton decides (with its alpha = 2 fallback).
+        if constexpr (MODE == 1)
+          accepted = accepted &&
+                     reference_reaches(cv, alpha, double(astate[cell]), a.residual_tol);
         if (accepted) {
           branch = 2;
           first = true;

@@ -384,6 +384,59 @@ __device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X
   dg = Sn + S + ds;
 }
 
+// What a certified evaluation leaves behind for reference_reaches: the point
+// e it evaluated, G'(e), h = -G''(e) = sum n^2 / (f - e n) and an
+// r >= max |n| / (f - e n).
+struct EvalCurv {
+  double e, dg, h, r;
+};
+
+// True only if the reference Newton (EntropicStep.h:111-140), started at the
+// stored alpha s, provably converges to the root accepted at a_acc -- the
+// condition under which a solve from another start (the network's) keeps the
+// reference's outcome.  G is concave (G'' = -sum n^2 / (f - a n) < 0) with
+// G(0) = 0, so (0, amax) holds at most one other root alpha*, on the branch
+// where G' < 0, and G' decreases.  From the data of one evaluation at e, for
+// every a with |a - e| r < 1: f - a n >= (f - e n)(1 - |a - e| r) > 0 (every
+// logarithm defined) and |G''(a)| <= h / (1 - |a - e| r).  With H that bound
+// over the points within D of e (D covers s, a_acc and a margin for delta):
+//  * G'(s) <= G'(e) + max(e - s, 0) H =: gps, and likewise gpa >= G'(a_acc).
+//    gps < 0 puts s right of G's maximum, so the reference's iterates do not
+//    head for the trivial root (from the left of the maximum the first step
+//    lands at or below 0 and the solve falls back to alpha = 2,
+//    Collision.h:344-346).
+//  * |G(a_acc)| <= tol and |G'| >= min(-gps, -gpa) between a_acc and alpha*
+//    wherever it matters below, so |a_acc - alpha*| <= delta = tol / that.
+//  * s >= a_acc + delta gives G(s) <= tol + (s - a_acc) gpa < 0: s is right
+//    of the root, and the iterates decrease monotonically to it inside
+//    [alpha*, s], where every log is defined.
+//  * otherwise, with d = alpha* - s > 0, Newton's error identity
+//    s1 - alpha* = d^2 (G''(x2) - G''(x1)/2) / G'(s) bounds the first step's
+//    overshoot by d^2 H / |G'(s)|; the iterates then decrease monotonically,
+//    and the logs stay defined if the overshoot point is within 1/r of e.
+// Quadratic convergence then meets |step| <= 1e-8 well inside the 50
+// iterations.  Relative allowances of 1e-6 cover the rounding of the bounds.
+__device__ __forceinline__ bool reference_reaches(const EvalCurv& c, double a_acc, double s,
+                                                  double tol) {
+  if (!(c.dg < 0.0)) return false;
+  // The bounds hold within D = D0 + m of e, m (at most 1) the margin left for
+  // a_acc +- delta, checked once delta is known.
+  const double D0 = fmax(fabs(s - c.e), fabs(a_acc - c.e));
+  const double lim = c.r > 0.0 ? (1.0 - 1e-3) / c.r : 2.0;
+  if (!(D0 < lim)) return false;
+  const double m = fmin(0.5 * (lim - D0), 1.0);
+  const double H = c.h / (1.0 - (D0 + m) * c.r) * (1.0 + 1e-6);
+  const double gps = c.dg + fmax(c.e - s, 0.0) * H;
+  const double gpa = c.dg + fmax(c.e - a_acc, 0.0) * H;
+  if (!(gps < 0.0 && gpa < 0.0)) return false;
+  const double delta = tol / fmin(-gps, -gpa) * (1.0 + 1e-6);
+  if (!(delta <= m)) return false;
+  if (s >= a_acc + delta) return true;
+  const double d = fmax(a_acc - s, 0.0) + delta;
+  const double far = fabs(a_acc + delta + d * d * H / -gps - c.e);
+  return far * c.r < 1.0 - 1e-3;
+}
+
 // One residual-checked Newton correction of a starting alpha (G in the log1p
 // form of EntropicStep.h:31-45).  Accepts the start if |G(start)| <= tol,
 // else the Newton point a1 = start - G/G' if it lies in (1, amax) and its
@@ -395,7 +448,8 @@ __device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X
 template <class L, class V>
 __device__ __forceinline__ bool residual_correction(const V& p, double amax,
                                                     double start, double tol,
-                                                    double& alpha, int& evals) {
+                                                    double& alpha, int& evals,
+                                                    EvalCurv& cv) {
   constexpr int Q = L::dimQ;
   evals = 0;
   if (!(start > 1.0 && start < amax)) return false;
@@ -423,6 +477,11 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
   double g, dg, h;
   residual_terms<L>(p, X, S, Sn, start, g, dg, h);
   evals = 1;
+  {
+    // |n| / (f - a n) = |x| / (1 - a x) <= xmax / (1 - a xmax)
+    const double den = 1.0 - start * X.xmax;
+    cv = EvalCurv{start, dg, h, den > 0.0 ? X.xmax / den : __builtin_inf()};
+  }
   if (fabs(g) <= tol) {
     alpha = start;
     return true;
@@ -467,7 +526,8 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
 template <class L, bool STRICT, bool FRESH, class V>
 __device__ __forceinline__ bool residual_newton_log(const V& p, double amax, double start,
                                                     double tol, int max_evals,
-                                                    double& alpha, int& evals) {
+                                                    double& alpha, int& evals,
+                                                    EvalCurv& cv) {
   constexpr int Q = L::dimQ;
   evals = 0;
   double SA = 0.0;
@@ -506,6 +566,7 @@ __device__ __forceinline__ bool residual_newton_log(const V& p, double amax, dou
       }
     });
     evals = it + 1;
+    cv = EvalCurv{a, dg, h, r};  // h, r: STRICT only
     if (fabs(g) <= tol) {
       if (!(a > 1.0)) return false;
       alpha = a;

@@ -164,6 +164,18 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
             a.set_alpha_mode(mode, tol)
             a.iterate(1)
             cold[mode] = a.elbm_stats()
+        # The stored alpha left of G's maximum (0.5): from there the reference
+        # Newton heads for the trivial root and falls back to alpha = 2
+        # (Collision.h:344-346), while one correction from the network's start
+        # reaches the nontrivial root.  The learned solver must keep the
+        # reference's outcome: it defers every cell whose root the reference
+        # Newton from the stored alpha is not shown to reach.
+        a.set_populations(f0)
+        a.set_alpha(np.full(a.N, 0.5))
+        a.reset_elbm_stats()
+        a.set_alpha_mode(AlphaMode.LEARNED, tol)
+        a.iterate(1, store=True)
+        left_alpha, left_stats = a.store_fields()[2], a.elbm_stats()
     fp, fneq = pulled_state(oracle, oc, f0)
     newton_cells, ref_fallbacks, worst, by_ref = 0, 0, 0.0, 0
     for k in range(fp.shape[1]):
@@ -191,6 +203,21 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
                 # reference Newton's iterate is within its last step (1e-8)
                 dg = abs(oracle.entropic_dg(int(lat), fp[:, k], fneq[:, k], aref))
                 assert abs(al - aref) <= 2.0 * tol / dg + 2e-8, (k, al, aref, dg)
+    left_fallbacks = 0
+    for k in range(fp.shape[1]):
+        aref, b, _ = oracle.elbm_alpha(int(lat), fp[:, k], fneq[:, k], 0.5)
+        if b in (oracle.ALPHA_SMALL, oracle.ALPHA_CAPPED):
+            continue
+        al = left_alpha[k]
+        if b != oracle.ALPHA_NEWTON:
+            left_fallbacks += 1
+            assert al == 2.0, (k, al)
+        else:
+            g = abs(oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], al))
+            assert g <= 1.001 * tol or abs(al - aref) <= 1e-9 * abs(aref), (k, al, aref, g)
+    say(f"stored alpha 0.5: {left_fallbacks} oracle fallbacks, learned {left_stats}")
+    assert left_fallbacks > 0.5 * newton_cells
+    assert left_stats["fallback"] == left_fallbacks
     say(f"{newton_cells} solved cells ({ref_fallbacks} oracle fallbacks); worst accepted "
         f"|G(alpha)| {worst:.3e} (tol {tol:.0e}); {by_ref} cell-solves left to the reference")
     assert newton_cells > 0.5 * fp.shape[1]

@@ -252,7 +252,8 @@ int lbm_reduce_noalloc(lbm_handle *h, double *mass, double *energy,
 /* ---- Bandwidth probes ------------------------------------------------------
  * Measurement aid, no reference counterpart: enqueues nsteps launches of a
  * probe that has the BGK vector kernel's memory access pattern -- 16 bytes
- * per lane, its tile order, per-row buffer resources, non-temporal stores --
+ * per lane, its tile order, per-row buffer resources, non-temporal loads
+ * and stores --
  * and none of its arithmetic, over the handle's two lattices (ping-pong), so
  * that the kernel's achieved bandwidth can be read against what the same
  * pattern reaches on this GPU.  Kinds:

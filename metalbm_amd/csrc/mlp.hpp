@@ -265,22 +265,20 @@ __device__ __forceinline__ double mlp_alpha_wave(const char* net, char* tile, co
 // a certificate that needs no logarithm.  A start is accepted when |G| <= tol
 // there, or its Newton point when the certificate is <= tol.
 
-// log1p(y) for |y| <= kSeriesMax by its Taylor series to degree 12
-// (truncation < |y|^13 / 13 < 1e-20 |y|): 12 FMAs instead of a library call.
-constexpr double kSeriesMax = 0.03;
-__device__ __forceinline__ double log1p_small(double y) {
-  double p = -1.0 / 12.0;
-  p = fma(p, y, 1.0 / 11.0);
-  p = fma(p, y, -1.0 / 10.0);
-  p = fma(p, y, 1.0 / 9.0);
-  p = fma(p, y, -1.0 / 8.0);
-  p = fma(p, y, 1.0 / 7.0);
-  p = fma(p, y, -1.0 / 6.0);
-  p = fma(p, y, 1.0 / 5.0);
-  p = fma(p, y, -1.0 / 4.0);
-  p = fma(p, y, 1.0 / 3.0);
-  p = fma(p, y, -1.0 / 2.0);
-  p = fma(p, y, 1.0);
+// log1p(y) by its Taylor series to degree D (Horner: D - 1 FMAs and a
+// multiply instead of a library call); truncation < |y|^(D+1) / ((D+1)(1 - |y|)).
+// The tiers below keep it under ~1e-19 |y|, a thousandth of an FP64 ulp:
+// degree 6 for |y| <= kTinyMax, 8 for <= kSmallMax, 12 for <= kSeriesMax.
+constexpr double kTinyMax = 9e-4;    // 9e-4^6 / 7 < 7.6e-20
+constexpr double kSmallMax = 5e-3;   // 5e-3^8 / 9 < 4.4e-20
+constexpr double kSeriesMax = 0.03;  // 0.03^12 / 13 < 4.1e-20
+template <int D>
+__device__ __forceinline__ double log1p_series(double y) {
+  double p = (D % 2 ? 1.0 : -1.0) / D;
+  static_for<D - 1>([&](auto jc) {
+    constexpr int k = D - 1 - decltype(jc)::value;
+    p = fma(p, y, (k % 2 ? 1.0 : -1.0) / k);
+  });
   return p * y;
 }
 // 1/d for d > 0 in the normal range: hardware reciprocal estimate plus two
@@ -320,6 +318,24 @@ __device__ __forceinline__ double log1p_mid(double y) {
   return 2.0 * z * s;
 }
 
+// Runs body(log1p_of) with the cheapest log1p variant that is exact (to the
+// bounds above) for every active lane of the wavefront: the lanes' largest
+// |y| decides, so the wavefront runs one variant rather than one per group
+// of lanes.  A NaN bound takes the library.
+template <class F>
+__device__ __forceinline__ void with_log1p(double ymax, F&& body) {
+  if (!__any(!(ymax <= kTinyMax)))
+    body([](double y) { return log1p_series<6>(y); });
+  else if (!__any(!(ymax <= kSmallMax)))
+    body([](double y) { return log1p_series<8>(y); });
+  else if (!__any(!(ymax <= kSeriesMax)))
+    body([](double y) { return log1p_series<12>(y); });
+  else if (!__any(!(ymax <= kMidMax)))
+    body([](double y) { return log1p_mid(y); });
+  else
+    body([](double y) { return log1p(y); });
+}
+
 // x_i = n_i / f_i for the certified correction: held per term for small Q,
 // recomputed at each use for D3Q27, where holding 27 FP64 values would cost
 // occupancy (the solve is latency-bound).
@@ -351,9 +367,9 @@ struct XTerms {
 //   G = a S - sum (f_i - a n_i) log1p(-t_i),  G' = Sn + S + sum n_i log1p(-t_i),
 //   -G'' = sum n_i x_i / (1 - t_i)
 // (f - a n = f (1 - t), ln((f - a n)/w) = A + log1p(-t)).  log1p(-t) by the
-// degree-12 Taylor series when max |t_i| = a max |x_i| <= kSeriesMax, by the
-// atanh series when <= kMidMax, else by the library; the choice is made once
-// per evaluation, not per term.  1/(1 - t) by the refined reciprocal.
+// Taylor series when max |t_i| = a max |x_i| <= kSeriesMax (degree by tier),
+// by the atanh series when <= kMidMax, else by the library; the choice is
+// made once per evaluation and wavefront, not per term (with_log1p).  1/(1 - t) by the refined reciprocal.
 template <class L, class V>
 __device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X,
                                                double S, double Sn, double a,
@@ -374,12 +390,7 @@ __device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X
       h += ni * xi * recip_refined(1.0 - t);
     });
   };
-  if (tmax <= kSeriesMax)
-    terms([](double y) { return log1p_small(y); });
-  else if (tmax <= kMidMax)
-    terms([](double y) { return log1p_mid(y); });
-  else
-    terms([](double y) { return log1p(y); });
+  with_log1p(tmax, terms);
   g = a * S - gs;
   dg = Sn + S + ds;
 }
@@ -468,12 +479,7 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
       Sn += ni;
     });
   };
-  if (ymax <= kSeriesMax)
-    sums([](double y) { return log1p_small(y); });
-  else if (ymax <= kMidMax)
-    sums([](double y) { return log1p_mid(y); });
-  else
-    sums([](double y) { return log1p(y); });
+  with_log1p(ymax, sums);
   double g, dg, h;
   residual_terms<L>(p, X, S, Sn, start, g, dg, h);
   evals = 1;
@@ -622,12 +628,7 @@ __device__ __forceinline__ double newton_residual(const V& p, double amax, doubl
       Sn += ni;
     });
   };
-  if (ymax <= kSeriesMax)
-    sums([](double y) { return log1p_small(y); });
-  else if (ymax <= kMidMax)
-    sums([](double y) { return log1p_mid(y); });
-  else
-    sums([](double y) { return log1p(y); });
+  with_log1p(ymax, sums);
   double a = start;
   for (int it = 1; it <= 50; ++it) {
     // the logarithm needs f_i - a fneq_i > 0 for every i: 0 <= a max x_i < 1

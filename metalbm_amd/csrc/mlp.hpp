@@ -345,11 +345,13 @@ struct XTerms {
   const V& p;
   double x[KEEP ? L::dimQ : 1];
   double xmax = 0.0;  // max_i |x_i|
+  double xpos = 0.0;  // max_i x_i, 0 if none positive: f - a n > 0 iff a xpos < 1 (a >= 0)
   __device__ __forceinline__ explicit XTerms(const V& p_) : p(p_) {
     static_for<L::dimQ>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const double xi = div_refined(p.template n<i>(), p.template f<i>());
       xmax = fmax(xmax, fabs(xi));
+      xpos = fmax(xpos, xi);
       if constexpr (KEEP) x[i] = xi;
     });
   }
@@ -517,8 +519,9 @@ __device__ __forceinline__ bool residual_correction(const V& p, double amax,
 // also a Newton point whose Taylor remainder bound is <= tol, without
 // evaluating G there (see below); returns false
 // when an iterate leaves the evaluation interval -- (1, amax) if STRICT (the
-// certified solvers), else (0, amax), where every f - a n > 0 so every
-// logarithm is defined -- or max_evals evaluations do not reach tol.
+// certified solvers), else a > 0 with every f - a n > 0 (outside, a
+// logarithm is NaN and so is G: the reference's loop fails there too) -- or
+// max_evals evaluations do not reach tol.
 // FP64 rounding of G is ~1e-15 absolute here (terms of order 1e-2), far below
 // the tolerances used.  Fewer live values than the log1p form of
 // residual_terms (no x_i, no series variants, no curvature), which keeps the
@@ -544,7 +547,10 @@ __device__ __forceinline__ bool residual_newton_log(const V& p, double amax, dou
   });
   double a = start;
   for (int it = 0; it < max_evals; ++it) {
-    if (!(a > (STRICT ? 1.0 : 0.0) && a < amax)) return false;
+    // STRICT keeps its iterates inside (1, amax); the plain Newton, like the
+    // reference's loop, bounds a only at convergence: an iterate outside the
+    // logarithms' domain gives a NaN G and ends in the fallback.
+    if (STRICT ? !(a > 1.0 && a < amax) : !(a > 0.0)) return false;
     // STRICT: each evaluation also gathers the curvature terms of the
     // certificate below, h = sum n^2 / fm and r = max |n| / fm, in the same
     // pass over the populations.
@@ -574,7 +580,7 @@ __device__ __forceinline__ bool residual_newton_log(const V& p, double amax, dou
     evals = it + 1;
     cv = EvalCurv{a, dg, h, r};  // h, r: STRICT only
     if (fabs(g) <= tol) {
-      if (!(a > 1.0)) return false;
+      if (!(a > 1.0 && a < amax)) return false;
       alpha = a;
       return true;
     }
@@ -631,8 +637,12 @@ __device__ __forceinline__ double newton_residual(const V& p, double amax, doubl
   with_log1p(ymax, sums);
   double a = start;
   for (int it = 1; it <= 50; ++it) {
-    // the logarithm needs f_i - a fneq_i > 0 for every i: 0 <= a max x_i < 1
-    if (!(a >= 0.0 && a * X.xmax < 1.0)) break;
+    // Every logarithm needs f_i - a fneq_i > 0: a xpos < 1 for a >= 0.  An
+    // iterate outside -- where the reference's G is NaN, or a < 0, from where
+    // its iterates converge to the trivial root -- ends in the fallback, as
+    // the reference's does; no bound on a before convergence, like the
+    // reference's loop.
+    if (!(a >= 0.0 && a * X.xpos < 1.0)) break;
     double g, dg, h;
     residual_terms<L>(p, X, S, Sn, a, g, dg, h);
     evals = it;

@@ -178,14 +178,31 @@ __device__ __forceinline__ bool elbm_prelude(const V& p, double& alpha,
     return false;
   }
   amax = 2.5;
-  static_for<Q>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    const double fi = p.template f<i>(), ni = p.template n<i>();
-    if (ni > 0) {
-      const double t = fabs(fi) / ni;
-      if (t < amax) amax = t;
-    }
-  });
+  if constexpr (REFDIV) {
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const double fi = p.template f<i>(), ni = p.template n<i>();
+      if (ni > 0) {
+        const double t = fabs(fi) / ni;
+        if (t < amax) amax = t;
+      }
+    });
+  } else {
+    // The smallest ratio by cross-multiplied comparisons (num / den < 2.5
+    // tracked as a pair) and one division at the end instead of Q: the
+    // products round, so two ratios within an ulp may be ordered either
+    // way, which moves amax by at most an ulp.
+    double num = 2.5, den = 1.0;
+    static_for<Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const double fi = fabs(p.template f<i>()), ni = p.template n<i>();
+      if (ni > 0 && fi * den < num * ni) {
+        num = fi;
+        den = ni;
+      }
+    });
+    if (den != 1.0 || num != 2.5) amax = num / den;
+  }
   if (amax < 2.) {
     branch = 1;
     alpha = 0.95 * amax;

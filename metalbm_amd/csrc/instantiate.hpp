@@ -30,7 +30,8 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
                        dim3(block), 0, st, a, fsx);
   else {
     // Modes 1 and 2: the certified correction, then the reference Newton for
-    // the cells it deferred (MODE 3, striding over the list).
+    // the cells it deferred (MODE 3, striding over the list; for D3Q27 mode 1
+    // MODE 5 comes between).
     hipLaunchKernelGGL((k_defer_reset<0>), dim3(1), dim3(64), 0, st, a.defer_count);
     if (a.alpha_mode == 2) {
       hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 2>), dim3(grid),
@@ -43,8 +44,23 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
     } else {
       return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>),
-                       dim3(std::min((a.N + 255) / 256, kDeferBlocks)), dim3(256), 0, st, a,
+    const dim3 list_grid(std::min((a.N + 255) / 256, kDeferBlocks));
+    if constexpr (kLogResidual<L> && MlpShape<L>::NIN > 0) {
+      if (a.alpha_mode == 1) {
+        // D3Q27 learned: the network's misses first get the stored-alpha
+        // solver (MODE 5, list 1 -> list 2), then the reference Newton
+        // takes what remains (MODE 3 over list 2).
+        hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 5>), dim3((a.N + 255) / 256),
+                           dim3(256), 0, st, a, fsx);
+        StepArgs b = a;
+        b.defer_count = a.defer_count + 1;
+        b.defer_list = a.defer_list2;
+        hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>), list_grid, dim3(256), 0, st,
+                           b, fsx);
+        return hipGetLastError();
+      }
+    }
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>), list_grid, dim3(256), 0, st, a,
                        fsx);
   }
   return hipGetLastError();

@@ -257,6 +257,14 @@ __device__ __forceinline__ double newton_reference(const V& p, double guess,
       ok = (x > 1. && x < amax);
       break;
     }
+    // A NaN step (an iterate outside the logarithms' domain) makes every
+    // later iterate NaN, so the reference's loop runs out its 50 iterations
+    // and falls back: end now with the same outcome and iteration count.
+    // (One such lane would otherwise hold its wavefront for all 50.)
+    if (step != step) {
+      iterations = 50;
+      break;
+    }
   }
   if (!ok) {
     branch = 3;

@@ -93,9 +93,8 @@ struct lbm_handle {
   bool ffield_built = false;
   uint64_t ffield_version = 0;  // fields_version the force field was built from
   // alpha modes 1 and 2: count and list of the cells the correction defers
-  unsigned* defer_count = nullptr;  // two counters
+  unsigned* defer_count = nullptr;
   int* defer_list = nullptr;
-  int* defer_list2 = nullptr;       // D3Q27 learned alpha only
   // datasets of the field file besides density and velocity
   // (writeAlpha, writeForce, writeVorticity; all on in Input_prod.in:68-72)
   int out_alpha = 1, out_force = 1, out_vorticity = 1;
@@ -280,7 +279,7 @@ void release(lbm_handle* h) {
                   (void*)h->staging, (void*)h->reduce_buf, h->ftab[0],
                   h->ftab[1], h->ftab[2], (void*)h->stats, (void*)h->mlp, h->mlp_packed,
                   (void*)h->mlp_m, (void*)h->mlp_v, (void*)h->defer_count,
-                  (void*)h->defer_list, (void*)h->defer_list2, (void*)h->log_x,
+                  (void*)h->defer_list, (void*)h->log_x,
                   (void*)h->log_alpha, (void*)h->log_count})
     if (p) (void)hipFree(p);
   if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
@@ -312,7 +311,6 @@ StepArgs make_args(const lbm_handle* h) {
   a.stats = h->stats;
   a.defer_count = h->defer_count;
   a.defer_list = h->defer_list;
-  a.defer_list2 = h->defer_list2;
   a.alpha_mode = h->alpha_mode;
   a.residual_tol = h->residual_tol;
   a.mlp_packed = h->mlp_packed;
@@ -848,10 +846,8 @@ int lbm_step(lbm_handle* h, int nsteps, int store_last) {
       (h->alpha_mode == LBM_ALPHA_LEARNED || h->alpha_mode == LBM_ALPHA_WARM_RESIDUAL)) {
     // each buffer on its own, so a failed second allocation leaks nothing
     // when the next call retries
-    if (!h->defer_count) HIP_TRY(h, hipMalloc(&h->defer_count, 2 * sizeof(unsigned)));
+    if (!h->defer_count) HIP_TRY(h, hipMalloc(&h->defer_count, sizeof(unsigned)));
     if (!h->defer_list) HIP_TRY(h, hipMalloc(&h->defer_list, (size_t)h->N * sizeof(int)));
-    if (h->cfg.lattice == LBM_D3Q27 && h->alpha_mode == LBM_ALPHA_LEARNED && !h->defer_list2)
-      HIP_TRY(h, hipMalloc(&h->defer_list2, (size_t)h->N * sizeof(int)));
   }
   StepArgs a = make_args(h);
   // Vector kernel: one wavefront per 64 V-cell segment, kVecWaves per workgroup.

@@ -290,11 +290,13 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 }
 
 // Entropic step, one instantiation per alpha solver MODE (0: reference
-// Newton; 1: network start, 2: stored-alpha start, each with one certified
-// correction and the reference Newton as fallback; 3: that fallback over the
+// Newton; 1: network start, 2: stored-alpha start, each with a certified
+// correction and plain Newton as fallback; 3: that fallback over the
 // deferred cells; 4: plain Newton from the stored alpha to the residual
-// tolerance; 5: MODE 2's solver over the cells MODE 1 deferred, D3Q27) so
-// that no kernel carries the registers of another's solver.
+// tolerance for every cell) so that no kernel carries the registers of
+// another's solver.  Plain Newton here is the reference's iteration stopped
+// at |G| <= residual_tol: it falls back to alpha = 2 exactly where the
+// reference Newton does and otherwise reaches the same root.
 // Three phases so that the learned mode's wavefront-wide matrix-core
 // inference runs with every lane present: (1) pull, moments, fneq and the
 // alpha prelude per cell; (2) MODE 1 only: the network predicts alpha for
@@ -315,11 +317,11 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // while the list is long.
 constexpr int kDeferBlocks = 2048;
 
-// Resets the two deferred-solve counters before the correction launch (a
-// kernel, not a memset, so that the step can be captured in a graph).
+// Resets the deferred-solve counter before the correction launch (a kernel,
+// not a memset, so that the step can be captured in a graph).
 template <int UNUSED = 0>
 __global__ void k_defer_reset(unsigned* count) {
-  if (threadIdx.x < 2) count[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) *count = 0u;
 }
 
 template <class L, int MODE>
@@ -348,26 +350,26 @@ __global__ void __launch_bounds__(BLOCK)
   char* wave_tile = s_lds + Lay::NET + (threadIdx.x >> 6) * Lay::TILE;
   if constexpr (MODE == 1)
     copy_packed_network<Shape::NIN, Shape::H1, Shape::H2>(a.mlp_packed, s_lds);
-  // MODE 3: the reference Newton for the cells the correction of MODE 1, 2
-  // or 5 deferred, one thread per listed cell, the workgroups striding over
-  // the list (a launch of at most kDeferBlocks workgroups: when nothing was
-  // deferred it costs little more than an empty launch).  MODE 5 reads the
-  // first list in one pass over a full grid (one thread per possible entry):
-  // the striding loop's state held its solver at 30 spilled registers
-  // instead of 4.  Every other mode makes one pass over its cells.
-  constexpr bool LISTED = MODE == 3 || MODE == 5;
-  constexpr bool STRIDED = MODE == 3;
-  const int n_listed = LISTED ? (int)*a.defer_count : 0;
-  // MODE 3 tallies its statistics per thread over the passes (branches 0-3,
-  // Newton iterations, accepted corrections) and adds them up once at the
-  // end, so that the passes need no barrier.
+  // MODE 3: plain Newton for the cells the correction of MODE 1 or 2
+  // deferred, one thread per listed cell, the workgroups striding over the
+  // list (a launch of at most kDeferBlocks workgroups: when nothing was
+  // deferred it costs little more than an empty launch).  Every other mode
+  // makes one pass.
+  // With the log-form plain Newton (Q > 9) the striding loop's state cost
+  // 23 spilled registers, so there MODE 3 makes one pass over a full grid
+  // (one thread per possible list entry) instead.
+  constexpr bool STRIDED = MODE == 3 && !kLogResidual<L>;
+  const int n_listed = MODE == 3 ? (int)*a.defer_count : 0;
+  // The striding MODE 3 tallies its statistics per thread over the passes
+  // (branches 0-3, Newton iterations, accepted corrections) and adds them up
+  // once at the end, so that the passes need no barrier.
   unsigned tally[6] = {0u, 0u, 0u, 0u, 0u, 0u};
   for (int list_base = blockIdx.x * BLOCK;; list_base += gridDim.x * BLOCK) {
-  if constexpr (LISTED) {
+  if constexpr (MODE == 3) {
     if (list_base >= n_listed) break;  // uniform over the workgroup
   }
   CellPos c;
-  if constexpr (LISTED) {
+  if constexpr (MODE == 3) {
     const int t = list_base + threadIdx.x;
     if (t < n_listed) {
       c = cell_from_index<L>(a, a.defer_list[t]);
@@ -439,9 +441,12 @@ __global__ void __launch_bounds__(BLOCK)
     C* __restrict__ astate = static_cast<C*>(a.alpha_state);
     if (solve) {
       const Fp64View<L, C, SH> p{f, fneq};
-      if constexpr (MODE == 0 || MODE == 3) {
+      if constexpr (MODE == 0) {
         alpha = newton_reference<L, REF>(p, double(astate[cell]), amax, branch, its);
-      } else if constexpr (MODE == 4) {
+      } else if constexpr (MODE == 4 || MODE == 3) {
+        // plain Newton from the stored alpha to the residual tolerance: the
+        // whole solve in MODE 4, the fallback of the cells MODE 1 or 2
+        // deferred in MODE 3
         if constexpr (kLogResidual<L>) {
           EvalCurv cv;
           branch = residual_newton_log<L, false, false>(p, amax, double(astate[cell]),
@@ -455,44 +460,44 @@ __global__ void __launch_bounds__(BLOCK)
         }
       } else {
         // MODE 1 starts from the network's alpha, MODE 2 from the stored one;
-        // if the solve below is not accepted the reference Newton decides --
-        // in the MODE 3 launch that follows, from the untouched stored alpha.
+        // if the solve below is not accepted plain Newton decides -- in the
+        // MODE 3 launch that follows, from the untouched stored alpha, with
+        // the reference's outcome.
         // MODE 1 makes one certified correction.  MODE 2 (D3Q27) may take a
         // second: from the stored alpha its iterates are the reference
         // Newton's own, so whatever it accepts is the root that Newton
         // converges to.  From the network's alpha a second step would also
         // reach roots from which the reference Newton, started at the stored
         // alpha, diverges to its alpha = 2 fallback.
-        // MODE 5 is MODE 2's solver over the cells MODE 1 deferred.
-        if constexpr (MODE == 2 || MODE == 5) start = double(astate[cell]);
+        if constexpr (MODE == 2) start = double(astate[cell]);
         int ev = 0;
         its = ev;
         bool accepted;
         EvalCurv cv;
         if constexpr (kLogResidual<L>)
-          accepted = residual_newton_log<L, true, MODE != 1>(p, amax, start, a.residual_tol,
+          accepted = residual_newton_log<L, true, MODE == 2>(p, amax, start, a.residual_tol,
                                                              MODE == 1 ? 1 : 2, alpha, ev, cv);
         else
           accepted = residual_correction<L>(p, amax, start, a.residual_tol, alpha, ev, cv);
         // The network's root stands only where the reference Newton from the
         // stored alpha provably reaches it; else the cell is deferred and
-        // that Newton decides (with its alpha = 2 fallback).
+        // plain Newton from the stored alpha decides (with the reference's
+        // alpha = 2 fallbacks).
         if constexpr (MODE == 1)
           accepted = accepted &&
                      reference_reaches(cv, alpha, double(astate[cell]), a.residual_tol);
         if (accepted) {
           branch = 2;
-          first = MODE != 5;
+          first = true;
         } else {
           deferred = true;  // listed below, once the wavefront has reconverged
         }
         its = ev;
       }
       // The sample log records the Newton solves: every solve in MODE 0 and
-      // MODE 4, the solves of the cells the correction deferred in MODES 5
-      // and 3 (the cells the network or the stored alpha missed -- data to
-      // retrain on).
-      if constexpr (Shape::NIN > 0 && (MODE == 0 || MODE == 3 || MODE == 4 || MODE == 5)) {
+      // MODE 4, the solves the correction deferred in MODE 3 (the cells the
+      // network or the stored alpha missed -- data to retrain on).
+      if constexpr (Shape::NIN > 0 && (MODE == 0 || MODE == 3 || MODE == 4)) {
         if (a.log_cap > 0 && branch == 2 &&
             sample_hash((unsigned)cell, a.log_salt) < a.log_threshold) {
           const unsigned long long k = atomicAdd(a.log_count, 1ull);
@@ -550,21 +555,18 @@ __global__ void __launch_bounds__(BLOCK)
     if (a.store) store_fields<L, C>(a, cell, FSX, rho, u, F, alpha);
     }
   }
-  if constexpr (MODE == 1 || MODE == 2 || MODE == 5) {
+  if constexpr (MODE == 1 || MODE == 2) {
     // One counter atomic per wavefront: the lanes' list slots follow from the
     // ballot (about a quarter of the cells defer in a turbulent D3Q27 run).
-    // MODES 1 and 2 append to the first list, MODE 5 (which reads it) to the
-    // second.
-    unsigned* const count = MODE == 5 ? a.defer_count + 1 : a.defer_count;
-    int* const list = MODE == 5 ? a.defer_list2 : a.defer_list;
     const unsigned long long mask = __ballot(deferred);
     if (mask) {
       const int lane = threadIdx.x & 63;
       const int leader = __ffsll((long long)mask) - 1;
       unsigned base = 0u;
-      if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(mask));
+      if (lane == leader) base = atomicAdd(a.defer_count, (unsigned)__popcll(mask));
       base = __shfl(base, leader);
-      if (deferred) list[base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull))] = cell;
+      if (deferred)
+        a.defer_list[base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull))] = cell;
     }
   }
   if (a.stats) {

@@ -29,9 +29,8 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
     hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 4>), dim3(grid),
                        dim3(block), 0, st, a, fsx);
   else {
-    // Modes 1 and 2: the certified correction, then the reference Newton for
-    // the cells it deferred (MODE 3, striding over the list; for D3Q27 mode 1
-    // MODE 5 comes between).
+    // Modes 1 and 2: the certified correction, then plain Newton for the
+    // cells it deferred (MODE 3, striding over the list).
     hipLaunchKernelGGL((k_defer_reset<0>), dim3(1), dim3(64), 0, st, a.defer_count);
     if (a.alpha_mode == 2) {
       hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 2>), dim3(grid),
@@ -44,24 +43,12 @@ hipError_t launch_step(const StepArgs& a, int fsx, int block, int grid,
     } else {
       return hipErrorInvalidValue;
     }
-    const dim3 list_grid(std::min((a.N + 255) / 256, kDeferBlocks));
-    if constexpr (kLogResidual<L> && MlpShape<L>::NIN > 0) {
-      if (a.alpha_mode == 1) {
-        // D3Q27 learned: the network's misses first get the stored-alpha
-        // solver (MODE 5, list 1 -> list 2), then the reference Newton
-        // takes what remains (MODE 3 over list 2).
-        hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 5>), dim3((a.N + 255) / 256),
-                           dim3(256), 0, st, a, fsx);
-        StepArgs b = a;
-        b.defer_count = a.defer_count + 1;
-        b.defer_list = a.defer_list2;
-        hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>), list_grid, dim3(256), 0, st,
-                           b, fsx);
-        return hipGetLastError();
-      }
-    }
-    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>), list_grid, dim3(256), 0, st, a,
-                       fsx);
+    // a capped grid striding over the list, or (log-form plain Newton) one
+    // thread per possible entry
+    const int full = (a.N + 255) / 256;
+    hipLaunchKernelGGL((k_elbm<L, S, C, FS, EQ, SH, 256, 3>),
+                       dim3(kLogResidual<L> ? full : std::min(full, kDeferBlocks)), dim3(256),
+                       0, st, a, fsx);
   }
   return hipGetLastError();
 }

@@ -64,13 +64,10 @@ struct StepArgs {
   unsigned log_threshold;  // sample when hash(cell, salt) < threshold
   unsigned log_salt;
   // Alpha modes 1 and 2: cells whose certified correction fails are listed
-  // here and solved by a second launch running the reference Newton, so the
-  // correction kernel does not carry the Newton's registers.  D3Q27 mode 1
-  // first passes its list through the stored-alpha solver (MODE 5), which
-  // lists its own failures in the second list.
-  unsigned* defer_count;  // two counters: the first list, the second
-  int* defer_list;        // N entries
-  int* defer_list2;       // N entries, D3Q27 mode 1 only
+  // here and solved by a second launch running plain Newton, so the
+  // correction kernel does not carry the Newton's registers.
+  unsigned* defer_count;
+  int* defer_list;  // N entries
 };
 
 // Storage / compute codes (values of LBM_F64 / LBM_F32 / LBM_F16).

@@ -83,22 +83,30 @@ enum {
   LBM_ALPHA_LEARNED = 1,  /* start from the MLP's alpha and make one Newton
                              correction: accept the start when |G| <=
                              residual_tol, or the corrected point when its
-                             Taylor remainder bound is <= residual_tol; else
-                             the reference Newton from the stored alpha
-                             decides (with its alpha = 2 fallback), in a
-                             second launch over the cells so deferred.  The
-                             alpha state is kept as for NEWTON. */
-  LBM_ALPHA_WARM_RESIDUAL = 2,  /* the same with the stored alpha as the start:
-                             the method without the network.  On D3Q27 it may
-                             take a second certified Newton step: from the
-                             stored alpha these are the reference Newton's own
-                             iterates, so the outcome stays the reference's */
+                             Taylor remainder bound is <= residual_tol --
+                             and only where a bound from the same evaluation
+                             shows the reference Newton from the stored alpha
+                             reaching that root; else plain Newton at the
+                             residual tolerance from the stored alpha decides
+                             (NEWTON_RESIDUAL's solver, whose alpha = 2
+                             fallbacks are the reference's), in a second
+                             launch over the cells so deferred.  The alpha
+                             state is kept as for NEWTON. */
+  LBM_ALPHA_WARM_RESIDUAL = 2,  /* the certified correction from the stored
+                             alpha: the method without the network (on D3Q27
+                             up to two certified steps).  Its iterates are
+                             plain Newton's, so it reaches the alpha
+                             NEWTON_RESIDUAL reaches; the certificate saves
+                             the last evaluation */
   LBM_ALPHA_NEWTON_RESIDUAL = 3  /* plain Newton at the residual tolerance: the
                              reference Newton (EntropicStep.h:111-140) from the
                              stored alpha, stopped when |G(alpha)| <=
                              residual_tol instead of |step| <= 1e-8, at most 50
                              evaluations, accepted inside (1, alphaMax), else
-                             alpha = 2 (Collision.h:328-349); one launch */
+                             alpha = 2 (Collision.h:328-349).  Like the
+                             reference's loop it bounds alpha only at
+                             convergence, so it falls back exactly where the
+                             reference Newton does; one launch */
 };
 
 typedef struct {

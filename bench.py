@@ -67,7 +67,8 @@ WORKLOADS = {
     "d2q9-elbm-2048-warm": dict(
         desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, the learned solver "
              "without its network: the stored alpha, one Newton correction accepted by "
-             "|G| <= 1e-10 or its Taylor certificate, else the reference Newton",
+             "|G| <= 1e-10 or its Taylor certificate, else plain Newton to the same "
+             "tolerance",
         cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
                  forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
                  wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True,
@@ -75,10 +76,11 @@ WORKLOADS = {
         init=("noise", 1e-3, 42), steps=200),
     "d2q9-elbm-2048-learned": dict(
         desc="D2Q9 SRT Entropic 2048^2 as d2q9-elbm-2048-newton, alpha from the MLP "
-             "(9 normalised non-equilibrium moments -> 32 -> 32 -> 1, FP32 weights) trained "
-             "on the GPU on (features, alpha) logged from the first 1000 Newton steps, then "
-             "one Newton correction accepted by |G| <= 1e-10 or its Taylor certificate, "
-             "else the reference Newton",
+             "(9 normalised non-equilibrium moments -> 32 -> 32 -> 1, FP32 master weights, "
+             "f16 matrix-core inference) trained on the GPU on (features, alpha) logged from "
+             "the first 1000 Newton steps, then one Newton correction accepted by |G| <= "
+             "1e-10 or its Taylor certificate where the reference Newton provably reaches "
+             "the same root, else plain Newton to the same tolerance",
         cfg=dict(lattice="D2Q9", collision="ELBM", nx=2048, ny=2048, nz=1, tau=0.50003,
                  forcing_scheme="Guo", force="Kolmogorov", amplitude=(1e-5, 0, 0),
                  wavelength=(512.0, 1, 1), storage="F32", compute="F32", shifted=True,
@@ -112,8 +114,8 @@ WORKLOADS = {
         init=("spectral", 0.1 / 3 ** 0.5, 16, 4.0, 42), steps=50),
     "d3q27-elbm-256-warm": dict(
         desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, the learned solver "
-             "without its network (stored alpha, one certified correction, else the "
-             "reference Newton)",
+             "without its network (stored alpha, up to two certified Newton steps, else "
+             "plain Newton to the same tolerance)",
         cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
                  forcing_scheme="None_", force="None_", storage="F32", compute="F32",
                  shifted=True, alpha_mode="WARM_RESIDUAL", residual_tol=1e-10),
@@ -122,8 +124,9 @@ WORKLOADS = {
         desc="D3Q27 SRT Entropic 256^3 as d3q27-elbm-256-newton, alpha from the MLP "
              "(27 normalised non-equilibrium moments -> 64 -> 64 -> 1, BF16 weights on the "
              "matrix cores, FP32 accumulate) trained on the GPU on (features, alpha) logged "
-             "from the first 200 Newton steps, one certified correction, else the reference "
-             "Newton",
+             "from the first 200 Newton steps, one certified correction kept where the "
+             "reference Newton provably reaches the same root, else plain Newton to the same "
+             "tolerance",
         cfg=dict(lattice="D3Q27", collision="ELBM", nx=256, ny=256, nz=256, tau=0.50006,
                  forcing_scheme="None_", force="None_", storage="F32", compute="F32",
                  shifted=True, alpha_mode="NEWTON", residual_tol=1e-10),

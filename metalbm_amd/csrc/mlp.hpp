@@ -455,9 +455,9 @@ __device__ __forceinline__ bool reference_reaches(const EvalCurv& c, double a_ac
 // else the Newton point a1 = start - G/G' if it lies in (1, amax) and its
 // Taylor certificate 1/2 s^2 max |G''| over the step is <= tol, with a
 // relative rounding allowance; max |G''| is bounded from the curvature at the
-// start (see below).  Returns false -- the caller then runs the
-// reference Newton -- when the start is outside (1, amax) or neither test
-// passes.  evals: G evaluations made (0 or 1).  p views f and fneq in FP64.
+// start (see below).  Returns false -- the caller then defers the cell to
+// plain Newton from the stored alpha -- when the start is outside (1, amax)
+// or neither test passes.  evals: G evaluations made (0 or 1).  p views f and fneq in FP64.
 template <class L, class V>
 __device__ __forceinline__ bool residual_correction(const V& p, double amax,
                                                     double start, double tol,

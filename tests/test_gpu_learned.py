@@ -193,7 +193,7 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
             g = abs(oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], al))
             if g <= 1.001 * tol:
                 worst = max(worst, g)
-            else:  # the reference Newton decided this cell
+            else:  # deferred, and plain Newton fell back as the reference does
                 assert abs(al - aref) <= 1e-9 * abs(aref), (k, mode, al, aref, g)
                 by_ref += 1
         # plain Newton at the residual tolerance: a root with |G| <= tol, or
@@ -291,8 +291,7 @@ def test_c4_fp32_build_every_alpha_mode(oracle):
     3. One step from the NEWTON state of step 100 in every mode, per cell:
        cells outside the oracle's Newton branch carry NEWTON's alpha
        bitwise (the prelude is shared); solved cells carry NEWTON's alpha
-       bitwise (the deferred reference Newton) or a root the residual
-       tolerance reaches from it, |alpha - alpha_N| <= 2 tol / |G'(alpha_N)| +
+       bitwise or a root the residual tolerance reaches from it, |alpha - alpha_N| <= 2 tol / |G'(alpha_N)| +
        2e-8 (alpha_N is within the reference Newton's last step, <= 1e-8, of
        the root; G' evaluated by the oracle)."""
     n, tol = 128, 1e-10
@@ -374,7 +373,7 @@ def test_c4_fp32_build_every_alpha_mode(oracle):
             borderline += any(one[m][0][k] != aN[k] for m in ALL_MODES[1:])
             continue
         if b != oracle.ALPHA_NEWTON or aN[k] == 2.0:
-            continue  # a fallback of the oracle or of the GPU's reference Newton
+            continue  # a fallback of the oracle or of the GPU's Newton
         nsolved += 1
         for mode in ALL_MODES[1:]:
             am = one[mode][0][k]

@@ -180,6 +180,17 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
             a.set_alpha_mode(mode, tol)
             a.iterate(1, store=True)
             left[mode] = (a.store_fields()[2], a.elbm_stats())
+        # Stored alphas spread over (0.3, 2.5): starts on both sides of G's
+        # maximum and of the root, where the guard must tell the two apart.
+        spread_alpha0 = np.random.default_rng(7).uniform(0.3, 2.5, a.N)
+        spread = {}
+        for mode in (AlphaMode.LEARNED, AlphaMode.WARM_RESIDUAL, AlphaMode.NEWTON_RESIDUAL):
+            a.set_populations(f0)
+            a.set_alpha(spread_alpha0)
+            a.reset_elbm_stats()
+            a.set_alpha_mode(mode, tol)
+            a.iterate(1, store=True)
+            spread[mode] = (a.store_fields()[2], a.elbm_stats())
     fp, fneq = pulled_state(oracle, oc, f0)
     newton_cells, ref_fallbacks, worst, by_ref = 0, 0, 0.0, 0
     for k in range(fp.shape[1]):
@@ -231,6 +242,27 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
     assert left_fallbacks > 0.5 * newton_cells
     for mode, (_, st) in left.items():
         assert st["fallback"] == left_fallbacks, mode
+    # the spread starts: every solver ends where the reference Newton from
+    # the same stored alpha ends
+    spread_fallbacks, spread_solved = 0, 0
+    for k in range(fp.shape[1]):
+        aref, b, _ = oracle.elbm_alpha(int(lat), fp[:, k], fneq[:, k], spread_alpha0[k])
+        if b in (oracle.ALPHA_SMALL, oracle.ALPHA_CAPPED):
+            continue
+        spread_fallbacks += b != oracle.ALPHA_NEWTON
+        spread_solved += b == oracle.ALPHA_NEWTON
+        for mode, (al_all, _) in spread.items():
+            al = al_all[k]
+            if b != oracle.ALPHA_NEWTON:
+                assert al == 2.0, (k, mode, al, spread_alpha0[k])
+            else:
+                g = abs(oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], al))
+                assert g <= 1.001 * tol or abs(al - aref) <= 1e-9 * abs(aref), \
+                    (k, mode, al, aref, g, spread_alpha0[k])
+    acc = spread[AlphaMode.LEARNED][1]["learned_accepted"]
+    say(f"spread stored alpha: {spread_fallbacks} oracle fallbacks, {spread_solved} solved; "
+        f"learned kept the network's root on {acc} cells")
+    assert spread_fallbacks > 0 and spread_solved > 0 and acc > 0
     say(f"{newton_cells} solved cells ({ref_fallbacks} oracle fallbacks); worst accepted "
         f"|G(alpha)| {worst:.3e} (tol {tol:.0e}); {by_ref} cell-solves left to the reference")
     assert newton_cells > 0.5 * fp.shape[1]

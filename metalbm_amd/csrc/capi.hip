@@ -58,7 +58,7 @@ struct lbm_handle {
   double* reduce_buf = nullptr;
   void* ftab[3] = {nullptr, nullptr, nullptr};
   int fax[3] = {3, 3, 3};
-  unsigned long long* stats = nullptr;  // 6 counters
+  unsigned long long* stats = nullptr;  // kStatSlots x kStatStride counters
   hipStream_t stream = nullptr;
   bool own_stream = false;
   StepFn step = nullptr;
@@ -518,8 +518,9 @@ int lbm_create(const lbm_config* cfg, lbm_handle** out) {
     const size_t ab = (size_t)h->N * (h->ctype == TYPE_F64 ? 8 : 4);
     e = hipMalloc(&h->alpha, ab);
     if (e == hipSuccess) e = launch_fill(h->ctype, h->alpha, 2.0, h->N, h->stream);  // Initialize.h:83-89
-    if (e == hipSuccess) e = hipMalloc(&h->stats, 6 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, 6 * sizeof(unsigned long long), h->stream);
+    const size_t sb = (size_t)kStatSlots * kStatStride * sizeof(unsigned long long);
+    if (e == hipSuccess) e = hipMalloc(&h->stats, sb);
+    if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, sb, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
       h->err = std::string("alpha field: ") + hipGetErrorString(e);
@@ -1016,9 +1017,13 @@ int lbm_get_elbm_stats(lbm_handle* h, lbm_elbm_stats* out) {
   if (!h || !out) return LBM_E_INVALID;
   std::memset(out, 0, sizeof(*out));
   if (!h->stats) return LBM_OK;
-  unsigned long long s[6];
-  HIP_TRY(h, hipMemcpyAsync(s, h->stats, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+  std::vector<unsigned long long> slots((size_t)kStatSlots * kStatStride);
+  HIP_TRY(h, hipMemcpyAsync(slots.data(), h->stats, slots.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, h->stream));
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  unsigned long long s[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = 0; k < kStatSlots; ++k)
+    for (int c = 0; c < 6; ++c) s[c] += slots[(size_t)k * kStatStride + c];
   out->small = (int64_t)s[0];
   out->capped = (int64_t)s[1];
   out->newton = (int64_t)s[2];
@@ -1031,9 +1036,9 @@ int lbm_get_elbm_stats(lbm_handle* h, lbm_elbm_stats* out) {
 int lbm_reset_elbm_stats(lbm_handle* h) {
   if (!h) return LBM_E_INVALID;
   // a fill kernel rather than a memset, so a caller may capture this call in
-  // a graph; +0.0 is the all-zero bit pattern of the six 64-bit counters
+  // a graph; +0.0 is the all-zero bit pattern of the 64-bit counters
   if (h->stats)
-    HIP_TRY(h, launch_fill(TYPE_F64, h->stats, 0.0, 6, h->stream));
+    HIP_TRY(h, launch_fill(TYPE_F64, h->stats, 0.0, kStatSlots * kStatStride, h->stream));
   return LBM_OK;
 }
 

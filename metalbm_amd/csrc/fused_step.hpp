@@ -599,7 +599,8 @@ __global__ void __launch_bounds__(BLOCK)
     }
     __syncthreads();
     if (threadIdx.x < 6 && s_cnt[threadIdx.x])
-      atomicAdd(&a.stats[threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+      atomicAdd(&a.stats[(blockIdx.x % kStatSlots) * kStatStride + threadIdx.x],
+                (unsigned long long)s_cnt[threadIdx.x]);
   }
 }
 

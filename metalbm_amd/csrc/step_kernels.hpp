@@ -28,6 +28,15 @@
 
 namespace lbm {
 
+// The six ELBM counters (small deviation, capped, solved, fallback, G
+// evaluations, accepted corrections) are kept in kStatSlots copies, one
+// 128-byte line each; workgroup b adds to copy b mod kStatSlots and the host
+// sums the copies.  With one copy every workgroup's atomics met on a single
+// cache line and serialised there: at 2048^2 D2Q9 (16 k workgroups) that,
+// not the arithmetic, set the entropic step's time.
+constexpr int kStatSlots = 256;
+constexpr int kStatStride = 16;  // counters per slot: 6 used, padded to 128 bytes
+
 struct StepArgs {
   const void* src;
   void* dst;
@@ -48,7 +57,7 @@ struct StepArgs {
   double* alpha_out;
   double* F_out;        // [D][N]
   void* alpha_state;    // ELBM warm-start field (compute type)
-  unsigned long long* stats;  // 6 ELBM counters or NULL
+  unsigned long long* stats;  // ELBM counters (kStatSlots slots) or NULL
   // Entropic alpha solver: 0 reference Newton; 1 network start, 2 stored
   // alpha start, each with one certified correction (|G| <= residual_tol or
   // Taylor bound), else the reference Newton; 3 plain Newton from the stored

@@ -433,20 +433,21 @@ __device__ __forceinline__ bool reference_reaches(const EvalCurv& c, double a_ac
                                                   double tol) {
   if (!(c.dg < 0.0)) return false;
   // The bounds hold within D = D0 + m of e, m (at most 1) the margin left for
-  // a_acc +- delta, checked once delta is known.
+  // a_acc +- delta, checked once delta is known.  Quotients by the refined
+  // reciprocal (a few ulp), well inside the 1e-6 allowances.
   const double D0 = fmax(fabs(s - c.e), fabs(a_acc - c.e));
-  const double lim = c.r > 0.0 ? (1.0 - 1e-3) / c.r : 2.0;
+  const double lim = c.r > 0.0 ? (1.0 - 1e-3) * recip_refined(c.r) : 2.0;
   if (!(D0 < lim)) return false;
   const double m = fmin(0.5 * (lim - D0), 1.0);
-  const double H = c.h / (1.0 - (D0 + m) * c.r) * (1.0 + 1e-6);
+  const double H = c.h * recip_refined(1.0 - (D0 + m) * c.r) * (1.0 + 1e-6);
   const double gps = c.dg + fmax(c.e - s, 0.0) * H;
   const double gpa = c.dg + fmax(c.e - a_acc, 0.0) * H;
   if (!(gps < 0.0 && gpa < 0.0)) return false;
-  const double delta = tol / fmin(-gps, -gpa) * (1.0 + 1e-6);
+  const double delta = tol * recip_refined(fmin(-gps, -gpa)) * (1.0 + 1e-6);
   if (!(delta <= m)) return false;
   if (s >= a_acc + delta) return true;
   const double d = fmax(a_acc - s, 0.0) + delta;
-  const double far = fabs(a_acc + delta + d * d * H / -gps - c.e);
+  const double far = fabs(a_acc + delta + d * d * H * recip_refined(-gps) - c.e);
   return far * c.r < 1.0 - 1e-3;
 }
 

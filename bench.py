@@ -622,9 +622,9 @@ def diagnostics_bench(K: int, W: int, n: int, cpu: bool) -> dict:
 def hdf5_bench(n: int) -> dict:
     """Field and distribution files in the reference's HDF5 layout (FP64,
     FFTW-padded last extent) for the D3Q19 n^3 FP32 workload: wall time of
-    lbm_write_fields_hdf5, lbm_write_distribution_hdf5 and
-    lbm_read_distribution_hdf5 into a temporary directory (device-host copy,
-    padding and libhdf5 included; the files are removed afterwards)."""
+    lbm_write_fields_hdf5 (first call and a later one), lbm_write_distribution_hdf5
+    and lbm_read_distribution_hdf5 into a temporary directory (device-host
+    copy, padding and libhdf5 included; the files are removed afterwards)."""
     import shutil
     import tempfile
 
@@ -639,15 +639,23 @@ def hdf5_bench(n: int) -> dict:
             init(a, wl["init"])
             a.iterate(2, store=True)
             a.synchronize()
+            # the first file call also loads libhdf5 and plans the vorticity
+            # transforms; the second is the steady cost
+            tf = time.perf_counter()
+            a.write_fields(d, 2)
+            first = time.perf_counter() - tf
+            a.iterate(1, store=True)
+            a.synchronize()
             t0 = time.perf_counter()
-            fpath = a.write_fields(d, 2)
+            fpath = a.write_fields(d, 3)
             t1 = time.perf_counter()
             dpath = a.write_distribution(d, 2)
             t2 = time.perf_counter()
             a.read_distribution(d, 2)
             t3 = time.perf_counter()
             fb, db = os.path.getsize(fpath), os.path.getsize(dpath)
-        out.update({"write_fields_s": round(t1 - t0, 3), "field_file_bytes": fb,
+        out.update({"write_fields_s": round(t1 - t0, 3), "first_write_fields_s": round(first, 3),
+                    "field_file_bytes": fb,
                     "write_distribution_s": round(t2 - t1, 3),
                     "read_distribution_s": round(t3 - t2, 3), "distribution_file_bytes": db,
                     "write_distribution_GBps": round(db / (t2 - t1) / 1e9, 2),

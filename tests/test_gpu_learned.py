@@ -184,7 +184,8 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
         # maximum and of the root, where the guard must tell the two apart.
         spread_alpha0 = np.random.default_rng(7).uniform(0.3, 2.5, a.N)
         spread = {}
-        for mode in (AlphaMode.LEARNED, AlphaMode.WARM_RESIDUAL, AlphaMode.NEWTON_RESIDUAL):
+        for mode in (AlphaMode.LEARNED, AlphaMode.WARM_RESIDUAL, AlphaMode.NEWTON_RESIDUAL,
+                     AlphaMode.NEWTON):
             a.set_populations(f0)
             a.set_alpha(spread_alpha0)
             a.reset_elbm_stats()
@@ -244,14 +245,20 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
         assert st["fallback"] == left_fallbacks, mode
     # the spread starts: every solver ends where the reference Newton from
     # the same stored alpha ends
-    spread_fallbacks, spread_solved = 0, 0
+    spread_fallbacks, spread_solved, spread_its = 0, 0, 0
     for k in range(fp.shape[1]):
-        aref, b, _ = oracle.elbm_alpha(int(lat), fp[:, k], fneq[:, k], spread_alpha0[k])
+        aref, b, its = oracle.elbm_alpha(int(lat), fp[:, k], fneq[:, k], spread_alpha0[k])
         if b in (oracle.ALPHA_SMALL, oracle.ALPHA_CAPPED):
             continue
         spread_fallbacks += b != oracle.ALPHA_NEWTON
         spread_solved += b == oracle.ALPHA_NEWTON
+        spread_its += its
+        # the reference Newton itself (FP64 here): the oracle's alpha
+        al = spread[AlphaMode.NEWTON][0][k]
+        assert abs(al - aref) <= 1e-9 * abs(aref), (k, al, aref, spread_alpha0[k])
         for mode, (al_all, _) in spread.items():
+            if mode == AlphaMode.NEWTON:
+                continue
             al = al_all[k]
             if b != oracle.ALPHA_NEWTON:
                 assert al == 2.0, (k, mode, al, spread_alpha0[k])
@@ -259,6 +266,11 @@ def test_learned_alpha_end_to_end(oracle, lat, shape, amp, lam):
                 g = abs(oracle.entropic_g(int(lat), fp[:, k], fneq[:, k], al))
                 assert g <= 1.001 * tol or abs(al - aref) <= 1e-9 * abs(aref), \
                     (k, mode, al, aref, g, spread_alpha0[k])
+    # its iteration count too -- a fallback whose iterate turned NaN ends at
+    # once but counts the reference's 50 iterations
+    st_n = spread[AlphaMode.NEWTON][1]
+    assert st_n["fallback"] == spread_fallbacks
+    assert st_n["newton_iterations"] == spread_its, (st_n, spread_its)
     acc = spread[AlphaMode.LEARNED][1]["learned_accepted"]
     say(f"spread stored alpha: {spread_fallbacks} oracle fallbacks, {spread_solved} solved; "
         f"learned kept the network's root on {acc} cells")

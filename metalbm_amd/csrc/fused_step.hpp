@@ -307,11 +307,12 @@ __device__ __forceinline__ unsigned sample_hash(unsigned cell, unsigned salt) {
 // D3Q27 workgroup pair at the 160 KiB limit).
 // Minimum wavefronts per SIMD the register allocator must leave room for
 // (the FP64 solve is latency-bound, so occupancy beats a few spills; measured
-// on one MI355X): D2Q9 learned three (168 VGPRs, 2 spilled; 17.3 k MLUPS vs
-// 13.8 k at two); D3Q27 learned three (168 VGPRs, none spilled; its launch
-// 3.81 -> 3.47 ms at 256^3 against two); the other D3Q27 solvers two
-// (reference Newton 2.72 k vs 2.03 k MLUPS at one wavefront with 289
-// registers; at three it spills 42 VGPRs).
+// on one MI355X at 2048^2 / 256^3): D2Q9 learned, stored-alpha and plain
+// Newton four (128 VGPRs with 2 and 4 spilled, 109 without: +10%, +12%,
+// +5% over three); D3Q27 learned three (168 VGPRs, none spilled; its launch
+// 3.81 -> 3.47 ms against two); the other D3Q27 solvers two (reference
+// Newton 2.72 k vs 2.03 k MLUPS at one wavefront with 289 registers; at
+// three it spills 42 VGPRs).
 // Workgroups of the MODE 3 launch: eight per CU, enough to keep every SIMD
 // busy at the entropic kernels' occupancy (two to four wavefronts per SIMD)
 // while the list is long.
@@ -326,7 +327,7 @@ __global__ void k_defer_reset(unsigned* count) {
 
 template <class L, int MODE>
 constexpr int elbm_min_waves() {
-  if (L::Type == LatticeType::D2Q9 && MODE == 1) return 3;
+  if (L::Type == LatticeType::D2Q9 && (MODE == 1 || MODE == 2 || MODE == 4)) return 4;
   if (L::Type == LatticeType::D3Q27 && MODE == 1) return 3;
   if (L::Type == LatticeType::D3Q27) return 2;
   return 1;

@@ -372,7 +372,10 @@ struct XTerms {
 // Taylor series when max |t_i| = a max |x_i| <= kSeriesMax (degree by tier),
 // by the atanh series when <= kMidMax, else by the library; the choice is
 // made once per evaluation and wavefront, not per term (with_log1p).  1/(1 - t) by the refined reciprocal.
-template <class L, class V>
+// FRESH (the iterating plain Newton) converts the populations afresh in
+// every evaluation, so the compiler holds no 2Q converted doubles across the
+// loop: 157 -> 109 VGPRs, four wavefronts per SIMD without spills.
+template <class L, bool FRESH = false, class V>
 __device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X,
                                                double S, double Sn, double a,
                                                double& g, double& dg, double& h) {
@@ -383,11 +386,18 @@ __device__ __forceinline__ void residual_terms(const V& p, const XTerms<L, V>& X
   auto terms = [&](auto log1p_of) {
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      const double ni = p.template n<i>();
+      double ni, fi;
+      if constexpr (FRESH) {
+        ni = p.template n_fresh<i>();
+        fi = p.template f_fresh<i>();
+      } else {
+        ni = p.template n<i>();
+        fi = p.template f<i>();
+      }
       const double xi = X.template get<i>();
       const double t = a * xi;
       const double l1 = log1p_of(-t);
-      gs += (p.template f<i>() - a * ni) * l1;
+      gs += (fi - a * ni) * l1;
       ds += ni * l1;
       h += ni * xi * recip_refined(1.0 - t);
     });
@@ -645,7 +655,7 @@ __device__ __forceinline__ double newton_residual(const V& p, double amax, doubl
     // reference's loop.
     if (!(a >= 0.0 && a * X.xpos < 1.0)) break;
     double g, dg, h;
-    residual_terms<L>(p, X, S, Sn, a, g, dg, h);
+    residual_terms<L, true>(p, X, S, Sn, a, g, dg, h);
     evals = it;
     if (fabs(g) <= tol) {
       if (a > 1.0 && a < amax) {

@@ -243,7 +243,16 @@ __device__ __forceinline__ double newton_reference(const V& p, double guess,
     double g = REFORDER ? 0.0 : SA, dg = 0.0;
     static_for<Q>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      const double fi = p.template f<i>(), ni = p.template n<i>();
+      // outside the parity build, converted afresh per iteration (no 2Q
+      // doubles held across the loop)
+      double fi, ni;
+      if constexpr (REFORDER) {
+        fi = p.template f<i>();
+        ni = p.template n<i>();
+      } else {
+        fi = p.template f_fresh<i>();
+        ni = p.template n_fresh<i>();
+      }
       const double fm = fi - x * ni;
       const double lg = log(fm / L::W[i]);
       if constexpr (REFORDER)

@@ -524,7 +524,10 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
             out = []
             for k in range(nsteps // every):
                 if mode == m.AlphaMode.LEARNED and k == 0:
-                    a.sample_log_enable(500000, 1e-3, 1)
+                    # fractions chosen so the expected sample count fits the
+                    # capacity: the kept samples, and so the network, do not
+                    # depend on the order the wavefronts reach the log
+                    a.sample_log_enable(500000, min(1.0, 2.5e-4 * (256 / n) ** 3), 1)
                     a.iterate(every, store=True)
                     a.sample_log_disable()
                     a.mlp_train(3000, 4096, 2e-3, seed=1)
@@ -537,7 +540,8 @@ def decay_study(nsteps: int, every: int, n: int) -> dict:
                         # log the deferred Newton solves, retrain every 5 intervals
                         if k > 1:
                             a.mlp_train(1000, 4096, 1e-3, seed=k)
-                        a.sample_log_enable(500000, 1e-2, k)
+                        # logging stays on for 5 intervals (500 steps)
+                        a.sample_log_enable(500000, min(1.0, 5e-5 * (256 / n) ** 3), k)
                     a.iterate(every, store=True)
                 out.append(a.reduce()[1:])
                 if (k + 1) % 10 == 0:

@@ -308,10 +308,14 @@ int lbm_mlp_gradient(lbm_handle *h, const float *x, const double *alpha, int n,
  * inference code (matrix cores: f16 operands for D2Q9, bf16 for D3Q27, FP32
  * accumulation, FP32 biases). */
 int lbm_mlp_predict(lbm_handle *h, const float *x, int n, double *alpha);
-/* Logs (features, alpha) of a `fraction` of the cells the reference Newton
+/* Logs (features, alpha) of a `fraction` of the cells the Newton solver
  * solves during subsequent steps, up to `capacity` samples, into device
- * memory: every solve in LBM_ALPHA_NEWTON, in the other modes the solves of
- * the cells their correction could not certify (data to retrain on). */
+ * memory: every solve in LBM_ALPHA_NEWTON and LBM_ALPHA_NEWTON_RESIDUAL, in
+ * the certified modes the solves of the cells their correction deferred
+ * (data to retrain on).  Which cells are sampled is a fixed hash of the cell,
+ * the salt and the step; when more solves qualify than `capacity` holds, the
+ * ones kept depend on the order the wavefronts reach the log, so choose
+ * `fraction` for the expected count to fit when runs must repeat exactly. */
 int lbm_sample_log_enable(lbm_handle *h, int64_t capacity, double fraction,
                           uint32_t salt);
 int lbm_sample_log_disable(lbm_handle *h);

@@ -181,17 +181,31 @@ __device__ inline double block_sum(double v, double* lds) {
   return lds[0];
 }
 
-// partial[b] = sum over the cells b, b + G, ... of 1/2 sum_c w_c^2.
+// partial[b] = sum over the cells of block b's threads (grid-stride) of
+// 1/2 sum_c w_c^2, one component after another.  Eight grid-stride loads are
+// issued before they are summed (in order), so each lane keeps eight reads in
+// flight instead of one.
 __global__ void k_half_square_partial(const double* __restrict__ w, int C, int64_t N,
                                       double* __restrict__ partial) {
   __shared__ double lds[kBlock];
+  constexpr int kU = 8;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   double acc = 0.0;
-  for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < N;
-       x += (int64_t)gridDim.x * blockDim.x)
-    for (int c = 0; c < C; ++c) {
-      const double v = w[c * N + x];
+  for (int c = 0; c < C; ++c) {
+    const double* __restrict__ wc = w + c * N;
+    int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; x + (kU - 1) * stride < N; x += kU * stride) {
+      double v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) v[u] = wc[x + u * stride];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) acc += 0.5 * v[u] * v[u];
+    }
+    for (; x < N; x += stride) {
+      const double v = wc[x];
       acc += 0.5 * v * v;
     }
+  }
   const double t = block_sum(acc, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
@@ -219,6 +233,7 @@ __device__ inline int shell_of(long long k2, bool physical) {
 
 constexpr int kShellWindow = 2048;  // shells binned per pass (LDS: 4 waves x 16 KB)
 constexpr int kShellBlocks = 4096;  // 1024 left the binning latency-bound (171 us at 256^3)
+constexpr int kShellBatch = 4;     // grid-stride iterations whose loads are issued together
 
 // Shell sums of c(m) sum_d |hat_d(m)|^2 over the shells [s_lo, s_lo + cnt),
 // c = 1/2 on the k_last = 0 plane and, when physical, on the k_last = n/2
@@ -237,57 +252,100 @@ __global__ void __launch_bounds__(kBlock) k_shell_bins(const double2* __restrict
   __syncthreads();
   const int lane = threadIdx.x & 63;
   double* mine = bins + (threadIdx.x >> 6) * cnt;
-  const int nlast = g.n[g.D - 1];
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < g.M;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = base + threadIdx.x;
-    int key = -1;
-    double val = 0.0;
-    if (m < g.M) {
-      const ModeIndex ix = mode_index(m, g.D, g.n);
-      long long k2 = 0;
-      for (int d = 0; d < g.D; ++d) {
-        const long long k = wavenumber(ix.i[d], g.n[d]);
-        k2 += k * k;
-      }
-      const int b = shell_of(k2, physical != 0) - s_lo;
-      if (b >= 0 && b < cnt) {
-        const int l = ix.i[g.D - 1];
-        const double c =
-            (l == 0 || (physical && (nlast & 1) == 0 && l == nlast / 2)) ? 0.5 : 1.0;
-        double e = 0.0;
-        for (int d = 0; d < g.D; ++d) {
-          const double2 v = hat[(int64_t)d * g.M + m];
-          e += v.x * v.x;
-          e += v.y * v.y;
+  const int nlast = g.D == 3 ? g.n[2] : g.n[1];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // kShellBatch grid-stride iterations at a time: every lane issues the loads
+  // of all of them before the first scan, so the scans and LDS adds of one
+  // iteration overlap the reads of the next.  Iterations are still binned in
+  // grid-stride order, so the sums are those of a one-at-a-time loop.
+  for (int64_t base0 = (int64_t)blockIdx.x * blockDim.x; base0 < g.M;
+       base0 += kShellBatch * stride) {
+    int keys[kShellBatch];
+    double wts[kShellBatch];
+    int rows[kShellBatch];  // row of the last axis (mode index / (n_last/2 + 1))
+    double2 hv[kShellBatch][3];
+    // loads first (from an in-range mode for lanes past the end or outside
+    // the window, whose values are then discarded), no use of them here
+#pragma unroll
+    for (int u = 0; u < kShellBatch; ++u) {
+      const int64_t m = base0 + u * stride + threadIdx.x;
+      keys[u] = -1;
+      wts[u] = 0.0;
+      rows[u] = -1;
+      const int64_t ml = m < g.M ? m : 0;
+      if (m < g.M) {
+        // mode_index and |k|^2 spelled out per dimension: indexing g.n and
+        // the index triple by a runtime axis put them in scratch memory
+        const unsigned mu = (unsigned)m, nh = (unsigned)(nlast / 2 + 1);
+        const int l = (int)(mu % nh);
+        const unsigned rest = mu / nh;
+        rows[u] = (int)rest;
+        long long k2;
+        if (g.D == 3) {
+          const long long k0 = wavenumber((int)(rest / (unsigned)g.n[1]), g.n[0]);
+          const long long k1 = wavenumber((int)(rest % (unsigned)g.n[1]), g.n[1]);
+          const long long k2l = wavenumber(l, nlast);
+          k2 = k0 * k0 + k1 * k1 + k2l * k2l;
+        } else {
+          const long long k0 = wavenumber((int)rest, g.n[0]);
+          const long long k1 = wavenumber(l, nlast);
+          k2 = k0 * k0 + k1 * k1;
         }
-        key = b;
-        val = c * e;
+        const int b = shell_of(k2, physical != 0) - s_lo;
+        if (b >= 0 && b < cnt) {
+          keys[u] = b;
+          wts[u] = (l == 0 || (physical && (nlast & 1) == 0 && l == nlast / 2)) ? 0.5 : 1.0;
+        }
       }
-    }
-    // Consecutive modes along the last axis fall in nondecreasing shells, so
-    // the wavefront's keys form runs: a segmented scan (fixed shuffle order)
-    // sums each run into its last lane, and only those lanes add to the bins.
-    const int prev_key = __shfl_up(key, 1);
-    const bool head = lane == 0 || prev_key != key;
-    int start = head ? lane : 0;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int o = __shfl_up(start, off);
-      if (lane >= off && o > start) start = o;
+      hv[u][0] = hat[ml];
+      hv[u][1] = hat[g.M + ml];
+      hv[u][2] = g.D == 3 ? hat[2 * g.M + ml] : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const double o = __shfl_up(val, off);
-      if (lane - off >= start) val += o;
-    }
-    const int next_key = __shfl_down(key, 1);
-    const bool tail = lane == 63 || next_key != key;
-    unsigned long long todo = __ballot(tail && key >= 0);
-    while (todo) {
-      const int t = __ffsll((long long)todo) - 1;
-      todo &= todo - 1;
-      if (lane == t) atomicAdd(&mine[key], val);
+    for (int u = 0; u < kShellBatch; ++u) {
+      if (base0 + u * stride >= g.M) break;  // block-uniform
+      const int key = keys[u];
+      double e = 0.0;
+      e += hv[u][0].x * hv[u][0].x;
+      e += hv[u][0].y * hv[u][0].y;
+      e += hv[u][1].x * hv[u][1].x;
+      e += hv[u][1].y * hv[u][1].y;
+      if (g.D == 3) {
+        e += hv[u][2].x * hv[u][2].x;
+        e += hv[u][2].y * hv[u][2].y;
+      }
+      double val = key >= 0 ? wts[u] * e : 0.0;
+      // Consecutive modes along the last axis fall in nondecreasing shells,
+      // so the wavefront's keys form runs: a segmented scan (fixed shuffle
+      // order) sums each run into its last lane, and only those lanes add to
+      // the bins.
+      const int prev_key = __shfl_up(key, 1);
+      const bool head = lane == 0 || prev_key != key;
+      int start = head ? lane : 0;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(start, off);
+        if (lane >= off && o > start) start = o;
+      }
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const double o = __shfl_up(val, off);
+        if (lane - off >= start) val += o;
+      }
+      const int next_key = __shfl_down(key, 1);
+      const bool tail = (lane == 63 || next_key != key) && key >= 0;
+      // Within one row of the last axis the keys are nondecreasing, so its
+      // run tails hold distinct shells: each row's tails add in one LDS
+      // instruction, rows in lane order -- every bin receives its adds in the
+      // same order as one lane at a time.
+      unsigned long long todo = __ballot(tail);
+      while (todo) {
+        const int t = __ffsll((long long)todo) - 1;
+        const int row_t = __builtin_amdgcn_readlane(rows[u], t);
+        const bool mine_now = tail && rows[u] == row_t;
+        todo &= ~__ballot(mine_now);
+        if (mine_now) atomicAdd(&mine[key], val);
+      }
     }
   }
   __syncthreads();

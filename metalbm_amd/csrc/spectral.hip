@@ -67,10 +67,14 @@ struct State {
   // direct transform of a small removal shell (spectral forces)
   bool direct_checked = false, direct = false;
   int nmodes = 0, dft_blocks = 0;
-  int4* modes = nullptr;          // (kx, ky, kz, Hermitian weight)
+  int npadded = 0;                // nmodes rounded up to whole passes
+  int dft_rows = 0;               // 2D small shells: row groups R of the column walk
+  bool small_k = false;           // every |k_d| of the modes <= 2
+  int4* modes = nullptr;          // (kx, ky, kz, Hermitian weight), zero-padded
   double2* tw = nullptr;          // e^{-2 pi i m / n} per axis: [nx] [ny] [nz]
   double2* dft_partial = nullptr; // [dft_blocks][nmodes][D]
-  double2* coef = nullptr;        // [nmodes][D]
+  double2* coef = nullptr;        // [npadded][D], zero beyond nmodes
+  double2* dense = nullptr;       // 2D small shells: [5][3][2] coefficient grid
   double* finj = nullptr;         // injection field [D][N] (Turbulent2D)
 };
 
@@ -81,7 +85,8 @@ void free_state(State* s) {
   if (s->have_inv_d) hipfftDestroy(s->inv_d);
   for (void* p : {(void*)s->real, (void*)s->hat, (void*)s->what, (void*)s->omega,
                   (void*)s->spec, (void*)s->partial, (void*)s->bins, (void*)s->modes,
-                  (void*)s->tw, (void*)s->dft_partial, (void*)s->coef, (void*)s->finj})
+                  (void*)s->tw, (void*)s->dft_partial, (void*)s->coef, (void*)s->dense,
+                  (void*)s->finj})
     if (p) (void)hipFree(p);
   delete s;
 }
@@ -483,10 +488,26 @@ __device__ inline double2 axis_phase(double2 e1, double2 e2, int k) {
   return k < 0 ? make_double2(r.x, -r.y) : r;
 }
 
-// e^{-i k.x} of mode m (signed wavenumbers) at the cell.
+// The same for |k| <= 2 by selects on the uniform k, no branches.
+__device__ inline double2 axis_phase_small(double2 e1, double2 e2, int k) {
+  const int a = k < 0 ? -k : k;
+  double2 r = a == 1 ? e1 : e2;
+  r = a == 0 ? make_double2(1.0, 0.0) : r;
+  return make_double2(r.x, k < 0 ? -r.y : r.y);
+}
+
+// e^{-i k.x} of mode m (signed wavenumbers) at the cell; SMALL when every
+// |k_d| of the modes is at most 2.
+template <bool SMALL>
 __device__ inline double2 phase_of(const CellPhases& c, const Dims& g, int4 m) {
-  double2 p = cmul(axis_phase(c.e[0], c.e2[0], m.x), axis_phase(c.e[1], c.e2[1], m.y));
-  if (g.D == 3) p = cmul(p, axis_phase(c.e[2], c.e2[2], m.z));
+  auto ax = [](double2 e1, double2 e2, int k) {
+    if constexpr (SMALL)
+      return axis_phase_small(e1, e2, k);
+    else
+      return axis_phase(e1, e2, k);
+  };
+  double2 p = cmul(ax(c.e[0], c.e2[0], m.x), ax(c.e[1], c.e2[1], m.y));
+  if (g.D == 3) p = cmul(p, ax(c.e[2], c.e2[2], m.z));
   return p;
 }
 
@@ -507,7 +528,9 @@ __device__ inline void cell_coords(int64_t x64, const Dims& g, int* i) {
 // partial[block][j][d] = sum over the block's cells of rho u_d e^{-i k_j.x},
 // modes j of this pass (blockIdx.y); fixed order per launch shape.  All
 // accumulator indices are compile-time constants (registers, not scratch).
-template <int D, int MP = modes_per_pass<D>()>
+// The mode list is padded to whole passes (k = 0, weight 0), so every pass
+// computes MP modes without a per-mode guard and writes the real ones.
+template <int D, bool SMALL, int MP = modes_per_pass<D>()>
 __global__ void __launch_bounds__(kBlock) k_mode_dft_partial(
     const double* __restrict__ rho, const double* __restrict__ u, Dims g, int64_t N,
     const int4* __restrict__ modes, int nmodes, const double2* __restrict__ tw,
@@ -532,7 +555,7 @@ __global__ void __launch_bounds__(kBlock) k_mode_dft_partial(
 #pragma unroll
     for (int j = 0; j < MP; ++j) {
       if (j < nm) {
-        const double2 e = phase_of(cp, g, modes[j0 + j]);
+        const double2 e = phase_of<SMALL>(cp, g, modes[j0 + j]);
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           acc[j][d].x += p[d] * e.x;
@@ -571,10 +594,109 @@ __global__ void __launch_bounds__(kBlock) k_mode_dft_partial(
   }
 }
 
+// 2D, every |k_d| <= 2: the same partial sums, accumulated per column.
+// Thread t < n1 * R walks column c = t % n1 over the rows t / n1, + R, ...;
+// on a column e^{-i ky theta1} is fixed, so the thread sums
+// B_kx,d = sum p_d e^{-i kx theta0} for kx = 0, 1, 2 only (p_d = rho u_d is
+// real, so B_-kx = conj(B_kx)) and multiplies by e^{-i ky theta1} per mode
+// at the end: 14 FP64 operations per cell instead of a phase product and
+// four FMAs per mode.  Consecutive threads read consecutive cells.
+template <int MP = modes_per_pass<2>()>
+__global__ void __launch_bounds__(kBlock) k_mode_dft_partial_2d_small(
+    const double* __restrict__ rho, const double* __restrict__ u, Dims g, int64_t N, int R,
+    const int4* __restrict__ modes, int nmodes, const double2* __restrict__ tw,
+    double2* __restrict__ partial) {
+  const int j0 = blockIdx.y * MP;
+  const int nm = min(MP, nmodes - j0);
+  const int n1 = g.n[1];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double b0[2] = {0.0, 0.0};
+  double2 b1[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  double2 b2[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+  int c = 0;
+  if (t < n1 * R) {
+    c = t % n1;
+    // four rows' loads issued together (rows still summed in order; rows
+    // past the end reload row r0 and are not summed)
+    constexpr int kRows = 4;
+    for (int r0 = t / n1; r0 < g.n[0]; r0 += kRows * R) {
+      double2 X1[kRows];
+      double rh[kRows], u0[kRows], u1[kRows];
+#pragma unroll
+      for (int q = 0; q < kRows; ++q) {
+        const int r = r0 + q * R < g.n[0] ? r0 + q * R : r0;
+        const int64_t x = (int64_t)r * n1 + c;
+        X1[q] = tw[r];
+        rh[q] = rho[x];
+        u0[q] = u[x];
+        u1[q] = u[N + x];
+      }
+#pragma unroll
+      for (int q = 0; q < kRows; ++q) {
+        if (r0 + q * R >= g.n[0]) break;
+        const double2 X2 = cmul(X1[q], X1[q]);
+        const double p[2] = {rh[q] * u0[q], rh[q] * u1[q]};
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          b0[d] += p[d];
+          b1[d].x += p[d] * X1[q].x;
+          b1[d].y += p[d] * X1[q].y;
+          b2[d].x += p[d] * X2.x;
+          b2[d].y += p[d] * X2.y;
+        }
+      }
+    }
+  }
+  const double2 Y1 = tw[g.n[0] + c];
+  const double2 Y2 = cmul(Y1, Y1);
+  double2 acc[MP][2];
+#pragma unroll
+  for (int j = 0; j < MP; ++j) {
+    const int4 m = modes[j0 + j];  // padded: in range for every j
+    const int a = m.x < 0 ? -m.x : m.x;
+    const double2 Y = m.y == 0 ? make_double2(1.0, 0.0) : (m.y == 1 ? Y1 : Y2);
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      double2 B = a == 0 ? make_double2(b0[d], 0.0) : (a == 1 ? b1[d] : b2[d]);
+      if (m.x < 0) B.y = -B.y;
+      acc[j][d] = t < n1 * R ? cmul(B, Y) : make_double2(0.0, 0.0);
+    }
+  }
+  __shared__ double2 red[kBlock / 64][MP][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < MP; ++j)
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      double re = acc[j][d].x, im = acc[j][d].y;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off);
+        im += __shfl_xor(im, off);
+      }
+      if (lane == 0) red[wave][j][d] = make_double2(re, im);
+    }
+  __syncthreads();
+  const int tt = threadIdx.x;
+  if (tt < nm * 2) {
+    const int j = tt / 2, d = tt - j * 2;
+    double2 sum = red[0][j][d];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+      sum.x += red[w][j][d].x;
+      sum.y += red[w][j][d].y;
+    }
+    partial[((int64_t)blockIdx.x * nmodes + j0 + j) * 2 + d] = sum;
+  }
+}
+
 // coef[j][d] = -R_d w_j p_hat_j,d / N: one block per (j, d), fixed order.
+// With dense non-null (2D, every |k_d| <= 2) the coefficient is also stored
+// at dense[((kx + 2) * 3 + ky) * 2 + d], ky >= 0 being the halved axis.
 __global__ void k_mode_dft_final(const double2* __restrict__ partial, int blocks, int nmodes,
                                  int D, const int4* __restrict__ modes, double R0, double R1,
-                                 double R2, double invN, double2* __restrict__ coef) {
+                                 double R2, double invN, double2* __restrict__ coef,
+                                 double2* __restrict__ dense) {
   __shared__ double lds[kBlock];
   const int t = blockIdx.x, j = t / D, d = t % D;
   double re = 0.0, im = 0.0;
@@ -590,13 +712,16 @@ __global__ void k_mode_dft_final(const double2* __restrict__ partial, int blocks
     const double R = d == 0 ? R0 : (d == 1 ? R1 : R2);
     const double w = -R * (double)modes[j].w * invN;
     coef[t] = make_double2(w * re, w * im);
+    if (dense) dense[((modes[j].x + 2) * 3 + modes[j].y) * 2 + d] = coef[t];
   }
 }
 
 // F_d(x) = finj_d(x) + sum_j Re(coef_j,d e^{+i k_j.x}); the weight in coef
 // (1 on the self-paired planes, 2 elsewhere) gives the complex-to-real
 // inverse of the half spectrum.
-template <class OUT>
+// The mode list and coef are padded to whole groups of modes_per_pass (zero
+// coefficients), so each group of modes is synthesised without a guard.
+template <class OUT, bool SMALL>
 __global__ void k_force_synth(const double* __restrict__ finj, const double2* __restrict__ coef,
                               const int4* __restrict__ modes, int nmodes,
                               const double2* __restrict__ tw, Dims g, int64_t N,
@@ -610,12 +735,11 @@ __global__ void k_force_synth(const double* __restrict__ finj, const double2* __
   double f[3] = {0.0, 0.0, 0.0};
   if (finj)
     for (int d = 0; d < g.D; ++d) f[d] = finj[d * N + x];
-  // constant trip count (uniform guard) so the mode and coefficient loads of
-  // all iterations can be issued ahead
+  constexpr int G = modes_per_pass<3>();
+  for (int j0 = 0; j0 < nmodes; j0 += G) {
 #pragma unroll
-  for (int j = 0; j < kMaxDirectModes; ++j) {
-    if (j < nmodes) {
-      const double2 e = phase_of(cp, g, modes[j]);  // e^{-i k.x}
+    for (int j = j0; j < j0 + G; ++j) {
+      const double2 e = phase_of<SMALL>(cp, g, modes[j]);  // e^{-i k.x}
 #pragma unroll
       for (int d = 0; d < 3; ++d)
         if (d < g.D) {
@@ -625,6 +749,47 @@ __global__ void k_force_synth(const double* __restrict__ finj, const double2* __
     }
   }
   for (int d = 0; d < g.D; ++d) F[d * N + x] = (OUT)f[d];
+}
+
+// 2D, every |k_d| <= 2: F_d = finj_d + Re sum_kx e^{+i kx theta0}
+// sum_ky C[kx][ky][d] e^{+i ky theta1} over the dense grid (zero where the
+// shell has no mode): compile-time harmonics, no per-mode selection.
+template <class OUT>
+__global__ void k_force_synth_2d_small(const double* __restrict__ finj,
+                                       const double2* __restrict__ dense,
+                                       const double2* __restrict__ tw, Dims g, int64_t N,
+                                       OUT* __restrict__ F) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N) return;
+  int i[3];
+  cell_coords(x, g, i);
+  // tw holds e^{-i theta}; the synthesis needs e^{+i k theta}
+  const double2 a1 = tw[i[0]], b1 = tw[g.n[0] + i[1]];
+  const double2 X1 = make_double2(a1.x, -a1.y), Y1 = make_double2(b1.x, -b1.y);
+  const double2 X2 = cmul(X1, X1), Y2 = cmul(Y1, Y1);
+  double f[2] = {finj ? finj[x] : 0.0, finj ? finj[N + x] : 0.0};
+#pragma unroll
+  for (int kx = -2; kx <= 2; ++kx) {
+    const double2 E = kx == 0 ? make_double2(1.0, 0.0)
+                              : (kx == 1 ? X1
+                                         : (kx == 2 ? X2
+                                                    : (kx == -1 ? make_double2(X1.x, -X1.y)
+                                                                : make_double2(X2.x, -X2.y))));
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      const double2 c0 = dense[((kx + 2) * 3 + 0) * 2 + d];
+      const double2 c1 = dense[((kx + 2) * 3 + 1) * 2 + d];
+      const double2 c2 = dense[((kx + 2) * 3 + 2) * 2 + d];
+      double gx = c0.x, gy = c0.y;
+      gx += c1.x * Y1.x - c1.y * Y1.y;
+      gy += c1.x * Y1.y + c1.y * Y1.x;
+      gx += c2.x * Y2.x - c2.y * Y2.y;
+      gy += c2.x * Y2.y + c2.y * Y2.x;
+      f[d] += gx * E.x - gy * E.y;  // Re(G E)
+    }
+  }
+  F[x] = (OUT)f[0];
+  F[N + x] = (OUT)f[1];
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -776,7 +941,20 @@ int direct_setup(lbm_handle* h, State& s, const lbm::ForceField& ff, const Force
   }
   if (modes.empty() || (int)modes.size() > kMaxDirectModes) return LBM_OK;
   s.nmodes = (int)modes.size();
-  s.dft_blocks = (int)std::min<int64_t>(kDftBlocks, (s.N + kBlock - 1) / kBlock);
+  s.small_k = true;
+  for (const int4& m : modes)
+    if (std::abs(m.x) > 2 || std::abs(m.y) > 2 || std::abs(m.z) > 2) s.small_k = false;
+  const int mp = modes_per_pass<3>();
+  s.npadded = (s.nmodes + mp - 1) / mp * mp;
+  modes.resize(s.npadded, make_int4(0, 0, 0, 0));
+  if (s.D == 2 && s.small_k) {
+    // n1 * R column walkers, about kDftBlocks blocks' worth, R <= n0
+    s.dft_rows = (int)std::max<int64_t>(
+        1, std::min<int64_t>(s.n[0], (int64_t)kDftBlocks * kBlock / s.n[1]));
+    s.dft_blocks = (int)(((int64_t)s.n[1] * s.dft_rows + kBlock - 1) / kBlock);
+  }
+  if (!(s.D == 2 && s.small_k))
+    s.dft_blocks = (int)std::min<int64_t>(kDftBlocks, (s.N + kBlock - 1) / kBlock);
   std::vector<double2> tw;
   for (int d = 0; d < s.D; ++d)
     for (int m = 0; m < s.n[d]; ++m) {
@@ -791,7 +969,15 @@ int direct_setup(lbm_handle* h, State& s, const lbm::ForceField& ff, const Force
   int rc = alloc((void**)&s.modes, sizeof(int4) * modes.size());
   if (!rc) rc = alloc((void**)&s.tw, sizeof(double2) * tw.size());
   if (!rc) rc = alloc((void**)&s.dft_partial, sizeof(double2) * s.dft_blocks * s.nmodes * s.D);
-  if (!rc) rc = alloc((void**)&s.coef, sizeof(double2) * s.nmodes * s.D);
+  if (!rc) rc = alloc((void**)&s.coef, sizeof(double2) * s.npadded * s.D);
+  if (!rc)
+    rc = lbm::record_hip(h, hipMemset(s.coef, 0, sizeof(double2) * s.npadded * s.D),
+                         "coefficient clear");
+  if (!rc && s.D == 2 && s.small_k) {
+    rc = alloc((void**)&s.dense, sizeof(double2) * 30);
+    if (!rc)
+      rc = lbm::record_hip(h, hipMemset(s.dense, 0, sizeof(double2) * 30), "coefficient clear");
+  }
   if (!rc && spec.injection) rc = alloc((void**)&s.finj, sizeof(double) * s.D * s.N);
   if (!rc)
     rc = lbm::record_hip(h, hipMemcpy(s.modes, modes.data(), sizeof(int4) * modes.size(),
@@ -836,21 +1022,36 @@ int lbm::update_spectral_force(lbm_handle* h) {
     const Dims g = dims_of(*s);
     const int mp = s->D == 3 ? modes_per_pass<3>() : modes_per_pass<2>();
     const unsigned passes = (unsigned)((s->nmodes + mp - 1) / mp);
-    if (s->D == 3)
-      hipLaunchKernelGGL(k_mode_dft_partial<3>, dim3(s->dft_blocks, passes), dim3(kBlock), 0, st,
-                         ff.rho, ff.u, g, s->N, s->modes, s->nmodes, s->tw, s->dft_partial);
-    else
-      hipLaunchKernelGGL(k_mode_dft_partial<2>, dim3(s->dft_blocks, passes), dim3(kBlock), 0, st,
-                         ff.rho, ff.u, g, s->N, s->modes, s->nmodes, s->tw, s->dft_partial);
+    if (s->D == 2 && s->small_k) {
+      hipLaunchKernelGGL(k_mode_dft_partial_2d_small<>, dim3(s->dft_blocks, passes), dim3(kBlock),
+                         0, st, ff.rho, ff.u, g, s->N, s->dft_rows, s->modes, s->nmodes, s->tw,
+                         s->dft_partial);
+    } else {
+      auto partial = s->D == 3 ? (s->small_k ? k_mode_dft_partial<3, true>
+                                             : k_mode_dft_partial<3, false>)
+                               : k_mode_dft_partial<2, false>;
+      hipLaunchKernelGGL(partial, dim3(s->dft_blocks, passes), dim3(kBlock), 0, st, ff.rho, ff.u,
+                         g, s->N, s->modes, s->nmodes, s->tw, s->dft_partial);
+    }
     hipLaunchKernelGGL(k_mode_dft_final, dim3(s->nmodes * s->D), dim3(kBlock), 0, st,
                        s->dft_partial, s->dft_blocks, s->nmodes, s->D, s->modes, spec.R[0],
-                       spec.R[1], spec.R[2], spec.scale, s->coef);
-    if (ff.f32)
-      hipLaunchKernelGGL(k_force_synth<float>, dim3(grid_of(s->N)), dim3(kBlock), 0, st, s->finj,
-                         s->coef, s->modes, s->nmodes, s->tw, g, s->N, ff.f32);
-    else
-      hipLaunchKernelGGL(k_force_synth<double>, dim3(grid_of(s->N)), dim3(kBlock), 0, st,
-                         s->finj, s->coef, s->modes, s->nmodes, s->tw, g, s->N, ff.f64);
+                       spec.R[1], spec.R[2], spec.scale, s->coef, s->dense);
+    if (s->dense) {
+      if (ff.f32)
+        hipLaunchKernelGGL(k_force_synth_2d_small<float>, dim3(grid_of(s->N)), dim3(kBlock), 0,
+                           st, s->finj, s->dense, s->tw, g, s->N, ff.f32);
+      else
+        hipLaunchKernelGGL(k_force_synth_2d_small<double>, dim3(grid_of(s->N)), dim3(kBlock), 0,
+                           st, s->finj, s->dense, s->tw, g, s->N, ff.f64);
+    } else if (ff.f32) {
+      auto synth = s->small_k ? k_force_synth<float, true> : k_force_synth<float, false>;
+      hipLaunchKernelGGL(synth, dim3(grid_of(s->N)), dim3(kBlock), 0, st, s->finj, s->coef,
+                         s->modes, s->npadded, s->tw, g, s->N, ff.f32);
+    } else {
+      auto synth = s->small_k ? k_force_synth<double, true> : k_force_synth<double, false>;
+      hipLaunchKernelGGL(synth, dim3(grid_of(s->N)), dim3(kBlock), 0, st, s->finj, s->coef,
+                         s->modes, s->npadded, s->tw, g, s->N, ff.f64);
+    }
     return lbm::record_hip(h, hipGetLastError(), "direct force kernels");
   }
   if (int rc = fft_force(h, *s, ff, spec, ff.f64)) return rc;

@@ -36,10 +36,19 @@ CASES = [
                                    force=ForceType.Turbulent2D, amplitude=(1e-2, 0, 1e-2),
                                    tau=0.8, **SPEC)),
     # removal shells of <= 32 modes take the direct transform (2D above);
-    # these cover the direct path in 3D and the full transforms in 2D
+    # these cover the direct path in 3D, the 2D path for |k_d| <= 2 and the
+    # full transforms in 2D
     ("d3q19-energy-removal-direct", Config(lattice=LatticeType.D3Q19, nx=10, ny=12, nz=8,
                                            force=ForceType.EnergyRemoval, tau=0.7,
                                            **{**SPEC, "removal_kmax": 2})),
+    # 2D shell with every |k_d| <= 2: the dense separable synthesis
+    ("d2q9-turbulent2d-direct-small", Config(lattice=LatticeType.D2Q9, nx=32, ny=24,
+                                             forcing_scheme=ForcingSchemeType.ExactDifferenceMethod,
+                                             force=ForceType.Turbulent2D, amplitude=(5e-3, 0, 0),
+                                             tau=0.6, **{**SPEC, "removal_kmax": 2})),
+    ("d2q9-energy-removal-direct-small", Config(lattice=LatticeType.D2Q9, nx=20, ny=18,
+                                                force=ForceType.EnergyRemoval, tau=0.7,
+                                                **{**SPEC, "removal_kmax": 2})),
     ("d2q9-turbulent2d-fft", Config(lattice=LatticeType.D2Q9, nx=30, ny=28,
                                     forcing_scheme=ForcingSchemeType.ExactDifferenceMethod,
                                     force=ForceType.Turbulent2D, amplitude=(5e-3, 0, 0),

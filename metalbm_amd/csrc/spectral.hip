@@ -525,6 +525,74 @@ __device__ inline void cell_coords(int64_t x64, const Dims& g, int* i) {
   }
 }
 
+// Wavefront sums of N values per lane (N a power of two, 2 <= N <= 64) by
+// recursive halving: at the step with lane offset o a lane keeps the half of
+// its values selected by its lane bit o, adding its partner's copy of that
+// half, and sends the other half; after log2 N steps each lane holds one
+// value summed over the lanes that differ from it in those bits, and xor
+// steps over the remaining low bits finish the sums.  Lane l returns the
+// wavefront sum of value l >> (6 - log2 N): N - 1 + 6 - log2 N shuffles
+// instead of 6 N.  Fixed order, so the sums are deterministic.
+template <int N>
+__device__ inline double wave_sums_halving(double (&v)[N], int lane) {
+  static_assert(N >= 2 && N <= 64 && (N & (N - 1)) == 0, "N: power of two <= 64");
+  int o = 32;
+#pragma unroll
+  for (int n = N; n > 1; n >>= 1, o >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int s = 0; s < n / 2; ++s) {
+      const double lo = v[s], hi = v[s + n / 2];
+      const double send = up ? lo : hi;
+      const double keep = up ? hi : lo;
+      v[s] = keep + __shfl_xor(send, o);
+    }
+  }
+  double r = v[0];
+#pragma unroll
+  for (int oo = o; oo > 0; oo >>= 1) r += __shfl_xor(r, oo);
+  return r;
+}
+
+// The block's sums of acc[j][d] (complex) for j < nm, written to
+// partial[block][j0 + j][d]: wavefront sums by recursive halving over the
+// MP * D * 2 reals (zero-padded to a power of two), then the wavefronts in
+// order.
+template <int MP, int D>
+__device__ inline void block_partials(const double2 (&acc)[MP][D], int nm, int nmodes, int j0,
+                                      double2* __restrict__ partial) {
+  constexpr int kVals = MP * D * 2;
+  constexpr int kN = kVals <= 32 ? 32 : 64;
+  static_assert(kVals <= 64, "at most 64 reals per lane");
+  constexpr int kShift = kN == 32 ? 1 : 0;  // lane l holds value l >> kShift
+  double v[kN];
+#pragma unroll
+  for (int j = 0; j < MP; ++j)
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      v[(j * D + d) * 2] = acc[j][d].x;
+      v[(j * D + d) * 2 + 1] = acc[j][d].y;
+    }
+#pragma unroll
+  for (int i = kVals; i < kN; ++i) v[i] = 0.0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double r = wave_sums_halving<kN>(v, lane);
+  __shared__ double red[kBlock / 64][kN];
+  if ((lane & ((1 << kShift) - 1)) == 0) red[wave][lane >> kShift] = r;
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < nm * D) {
+    const int j = t / D, d = t - j * D;
+    double2 sum = make_double2(red[0][(j * D + d) * 2], red[0][(j * D + d) * 2 + 1]);
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) {
+      sum.x += red[w][(j * D + d) * 2];
+      sum.y += red[w][(j * D + d) * 2 + 1];
+    }
+    partial[((int64_t)blockIdx.x * nmodes + j0 + j) * D + d] = sum;
+  }
+}
+
 // partial[block][j][d] = sum over the block's cells of rho u_d e^{-i k_j.x},
 // modes j of this pass (blockIdx.y); fixed order per launch shape.  All
 // accumulator indices are compile-time constants (registers, not scratch).
@@ -564,34 +632,9 @@ __global__ void __launch_bounds__(kBlock) k_mode_dft_partial(
       }
     }
   }
-  // wavefront butterfly sums (no barrier), then the block's wavefronts in
-  // order: one barrier instead of one per accumulator
-  __shared__ double2 red[kBlock / 64][MP][D];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < MP; ++j)
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      double re = acc[j][d].x, im = acc[j][d].y;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        re += __shfl_xor(re, off);
-        im += __shfl_xor(im, off);
-      }
-      if (lane == 0) red[wave][j][d] = make_double2(re, im);
-    }
-  __syncthreads();
-  const int t = threadIdx.x;
-  if (t < nm * D) {
-    const int j = t / D, d = t - j * D;
-    double2 sum = red[0][j][d];
-#pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w) {
-      sum.x += red[w][j][d].x;
-      sum.y += red[w][j][d].y;
-    }
-    partial[((int64_t)blockIdx.x * nmodes + j0 + j) * D + d] = sum;
-  }
+  // wavefront sums by recursive halving (no barrier), then the block's
+  // wavefronts in order: one barrier instead of one per accumulator
+  block_partials<MP, D>(acc, nm, nmodes, j0, partial);
 }
 
 // 2D, every |k_d| <= 2: the same partial sums, accumulated per column.
@@ -662,32 +705,7 @@ __global__ void __launch_bounds__(kBlock) k_mode_dft_partial_2d_small(
       acc[j][d] = t < n1 * R ? cmul(B, Y) : make_double2(0.0, 0.0);
     }
   }
-  __shared__ double2 red[kBlock / 64][MP][2];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int j = 0; j < MP; ++j)
-#pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      double re = acc[j][d].x, im = acc[j][d].y;
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        re += __shfl_xor(re, off);
-        im += __shfl_xor(im, off);
-      }
-      if (lane == 0) red[wave][j][d] = make_double2(re, im);
-    }
-  __syncthreads();
-  const int tt = threadIdx.x;
-  if (tt < nm * 2) {
-    const int j = tt / 2, d = tt - j * 2;
-    double2 sum = red[0][j][d];
-#pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w) {
-      sum.x += red[w][j][d].x;
-      sum.y += red[w][j][d].y;
-    }
-    partial[((int64_t)blockIdx.x * nmodes + j0 + j) * 2 + d] = sum;
-  }
+  block_partials<MP, 2>(acc, nm, nmodes, j0, partial);
 }
 
 // coef[j][d] = -R_d w_j p_hat_j,d / N: one block per (j, d), fixed order.
@@ -753,7 +771,10 @@ __global__ void k_force_synth(const double* __restrict__ finj, const double2* __
 
 // 2D, every |k_d| <= 2: F_d = finj_d + Re sum_kx e^{+i kx theta0}
 // sum_ky C[kx][ky][d] e^{+i ky theta1} over the dense grid (zero where the
-// shell has no mode): compile-time harmonics, no per-mode selection.
+// shell has no mode): compile-time harmonics, no per-mode selection.  The
+// kx and -kx terms are paired: with E = e^{+i k theta0}, G_k = sum_ky C Y,
+// Re(E G_k + conj(E) G_-k) = E.x Re(G_k + G_-k) - E.y Im(G_k - G_-k), and
+// G_k +- G_-k = sum_ky (C_k +- C_-k) Y, so each pair costs two real sums.
 template <class OUT>
 __global__ void k_force_synth_2d_small(const double* __restrict__ finj,
                                        const double2* __restrict__ dense,
@@ -767,26 +788,36 @@ __global__ void k_force_synth_2d_small(const double* __restrict__ finj,
   const double2 a1 = tw[i[0]], b1 = tw[g.n[0] + i[1]];
   const double2 X1 = make_double2(a1.x, -a1.y), Y1 = make_double2(b1.x, -b1.y);
   const double2 X2 = cmul(X1, X1), Y2 = cmul(Y1, Y1);
+  auto C = [&](int kx, int ky, int d) { return dense[((kx + 2) * 3 + ky) * 2 + d]; };
   double f[2] = {finj ? finj[x] : 0.0, finj ? finj[N + x] : 0.0};
 #pragma unroll
-  for (int kx = -2; kx <= 2; ++kx) {
-    const double2 E = kx == 0 ? make_double2(1.0, 0.0)
-                              : (kx == 1 ? X1
-                                         : (kx == 2 ? X2
-                                                    : (kx == -1 ? make_double2(X1.x, -X1.y)
-                                                                : make_double2(X2.x, -X2.y))));
+  for (int d = 0; d < 2; ++d) {
+    // kx = 0: Re G_0
+    const double2 c00 = C(0, 0, d), c01 = C(0, 1, d), c02 = C(0, 2, d);
+    double acc = c00.x;
+    acc += c01.x * Y1.x - c01.y * Y1.y;
+    acc += c02.x * Y2.x - c02.y * Y2.y;
 #pragma unroll
-    for (int d = 0; d < 2; ++d) {
-      const double2 c0 = dense[((kx + 2) * 3 + 0) * 2 + d];
-      const double2 c1 = dense[((kx + 2) * 3 + 1) * 2 + d];
-      const double2 c2 = dense[((kx + 2) * 3 + 2) * 2 + d];
-      double gx = c0.x, gy = c0.y;
-      gx += c1.x * Y1.x - c1.y * Y1.y;
-      gy += c1.x * Y1.y + c1.y * Y1.x;
-      gx += c2.x * Y2.x - c2.y * Y2.y;
-      gy += c2.x * Y2.y + c2.y * Y2.x;
-      f[d] += gx * E.x - gy * E.y;  // Re(G E)
+    for (int k = 1; k <= 2; ++k) {
+      const double2 E = k == 1 ? X1 : X2;
+      double re = 0.0, im = 0.0;  // Re(G_k + G_-k), Im(G_k - G_-k)
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky) {
+        const double2 cp = C(k, ky, d), cm = C(-k, ky, d);
+        const double px = cp.x + cm.x, py = cp.y + cm.y;  // C_k + C_-k
+        const double mx = cp.x - cm.x, my = cp.y - cm.y;  // C_k - C_-k
+        if (ky == 0) {
+          re += px;
+          im += my;
+        } else {
+          const double2 Y = ky == 1 ? Y1 : Y2;
+          re += px * Y.x - py * Y.y;
+          im += mx * Y.y + my * Y.x;
+        }
+      }
+      acc += E.x * re - E.y * im;
     }
+    f[d] += acc;
   }
   F[x] = (OUT)f[0];
   F[N + x] = (OUT)f[1];

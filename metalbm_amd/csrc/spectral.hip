@@ -326,12 +326,10 @@ __global__ void __launch_bounds__(kBlock) k_shell_bins(const double2* __restrict
       // the bins.
       const int prev_key = __shfl_up(key, 1);
       const bool head = lane == 0 || prev_key != key;
-      int start = head ? lane : 0;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(start, off);
-        if (lane >= off && o > start) start = o;
-      }
+      // the run's first lane: the highest head at or below this lane (lane 0
+      // is always a head; 2 << 63 wraps to 0, so lane 63 keeps every bit)
+      const unsigned long long heads = __ballot(head) & ((2ull << lane) - 1);
+      const int start = 63 - __clzll((long long)heads);
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
         const double o = __shfl_up(val, off);

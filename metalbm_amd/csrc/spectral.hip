@@ -778,47 +778,80 @@ __global__ void k_force_synth_2d_small(const double* __restrict__ finj,
                                        const double2* __restrict__ dense,
                                        const double2* __restrict__ tw, Dims g, int64_t N,
                                        OUT* __restrict__ F) {
-  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= N) return;
-  int i[3];
-  cell_coords(x, g, i);
-  // tw holds e^{-i theta}; the synthesis needs e^{+i k theta}
-  const double2 a1 = tw[i[0]], b1 = tw[g.n[0] + i[1]];
-  const double2 X1 = make_double2(a1.x, -a1.y), Y1 = make_double2(b1.x, -b1.y);
-  const double2 X2 = cmul(X1, X1), Y2 = cmul(Y1, Y1);
+  // two cells per lane, a block width apart: the pairing of the uniform
+  // coefficients below is shared by both
+  constexpr int kC = 2;
+  const int64_t x0 = (int64_t)blockIdx.x * (kC * blockDim.x) + threadIdx.x;
+  if (x0 >= N) return;
+  bool live[kC];
+  double2 X1[kC], X2[kC], Y1[kC], Y2[kC];
+  double f[kC][2];
+#pragma unroll
+  for (int q = 0; q < kC; ++q) {
+    const int64_t xq = x0 + q * blockDim.x;
+    live[q] = xq < N;
+    const int64_t x = live[q] ? xq : x0;
+    int i[3];
+    cell_coords(x, g, i);
+    // tw holds e^{-i theta}; the synthesis needs e^{+i k theta}
+    const double2 a1 = tw[i[0]], b1 = tw[g.n[0] + i[1]];
+    X1[q] = make_double2(a1.x, -a1.y);
+    Y1[q] = make_double2(b1.x, -b1.y);
+    X2[q] = cmul(X1[q], X1[q]);
+    Y2[q] = cmul(Y1[q], Y1[q]);
+    f[q][0] = finj ? finj[x] : 0.0;
+    f[q][1] = finj ? finj[N + x] : 0.0;
+  }
   auto C = [&](int kx, int ky, int d) { return dense[((kx + 2) * 3 + ky) * 2 + d]; };
-  double f[2] = {finj ? finj[x] : 0.0, finj ? finj[N + x] : 0.0};
 #pragma unroll
   for (int d = 0; d < 2; ++d) {
     // kx = 0: Re G_0
     const double2 c00 = C(0, 0, d), c01 = C(0, 1, d), c02 = C(0, 2, d);
-    double acc = c00.x;
-    acc += c01.x * Y1.x - c01.y * Y1.y;
-    acc += c02.x * Y2.x - c02.y * Y2.y;
+    double acc[kC];
+#pragma unroll
+    for (int q = 0; q < kC; ++q) {
+      acc[q] = c00.x;
+      acc[q] += c01.x * Y1[q].x - c01.y * Y1[q].y;
+      acc[q] += c02.x * Y2[q].x - c02.y * Y2[q].y;
+    }
 #pragma unroll
     for (int k = 1; k <= 2; ++k) {
-      const double2 E = k == 1 ? X1 : X2;
-      double re = 0.0, im = 0.0;  // Re(G_k + G_-k), Im(G_k - G_-k)
+      double re[kC], im[kC];  // Re(G_k + G_-k), Im(G_k - G_-k)
+#pragma unroll
+      for (int q = 0; q < kC; ++q) re[q] = im[q] = 0.0;
 #pragma unroll
       for (int ky = 0; ky < 3; ++ky) {
         const double2 cp = C(k, ky, d), cm = C(-k, ky, d);
         const double px = cp.x + cm.x, py = cp.y + cm.y;  // C_k + C_-k
         const double mx = cp.x - cm.x, my = cp.y - cm.y;  // C_k - C_-k
-        if (ky == 0) {
-          re += px;
-          im += my;
-        } else {
-          const double2 Y = ky == 1 ? Y1 : Y2;
-          re += px * Y.x - py * Y.y;
-          im += mx * Y.y + my * Y.x;
+#pragma unroll
+        for (int q = 0; q < kC; ++q) {
+          if (ky == 0) {
+            re[q] += px;
+            im[q] += my;
+          } else {
+            const double2 Y = ky == 1 ? Y1[q] : Y2[q];
+            re[q] += px * Y.x - py * Y.y;
+            im[q] += mx * Y.y + my * Y.x;
+          }
         }
       }
-      acc += E.x * re - E.y * im;
+#pragma unroll
+      for (int q = 0; q < kC; ++q) {
+        const double2 E = k == 1 ? X1[q] : X2[q];
+        acc[q] += E.x * re[q] - E.y * im[q];
+      }
     }
-    f[d] += acc;
+#pragma unroll
+    for (int q = 0; q < kC; ++q) f[q][d] += acc[q];
   }
-  F[x] = (OUT)f[0];
-  F[N + x] = (OUT)f[1];
+#pragma unroll
+  for (int q = 0; q < kC; ++q)
+    if (live[q]) {
+      const int64_t x = x0 + q * blockDim.x;
+      F[x] = (OUT)f[q][0];
+      F[N + x] = (OUT)f[q][1];
+    }
 }
 
 unsigned grid_of(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1067,11 +1100,11 @@ int lbm::update_spectral_force(lbm_handle* h) {
                        spec.R[1], spec.R[2], spec.scale, s->coef, s->dense);
     if (s->dense) {
       if (ff.f32)
-        hipLaunchKernelGGL(k_force_synth_2d_small<float>, dim3(grid_of(s->N)), dim3(kBlock), 0,
-                           st, s->finj, s->dense, s->tw, g, s->N, ff.f32);
+        hipLaunchKernelGGL(k_force_synth_2d_small<float>, dim3(grid_of((s->N + 1) / 2)),
+                           dim3(kBlock), 0, st, s->finj, s->dense, s->tw, g, s->N, ff.f32);
       else
-        hipLaunchKernelGGL(k_force_synth_2d_small<double>, dim3(grid_of(s->N)), dim3(kBlock), 0,
-                           st, s->finj, s->dense, s->tw, g, s->N, ff.f64);
+        hipLaunchKernelGGL(k_force_synth_2d_small<double>, dim3(grid_of((s->N + 1) / 2)),
+                           dim3(kBlock), 0, st, s->finj, s->dense, s->tw, g, s->N, ff.f64);
     } else if (ff.f32) {
       auto synth = s->small_k ? k_force_synth<float, true> : k_force_synth<float, false>;
       hipLaunchKernelGGL(synth, dim3(grid_of(s->N)), dim3(kBlock), 0, st, s->finj, s->coef,
